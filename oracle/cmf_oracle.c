@@ -1,0 +1,719 @@
+/*
+ * cmf_oracle.c -- CPU oracle for the pFq CMF walk (TEST INFRASTRUCTURE ONLY).
+ *
+ * The checker for the HIP walk engine; never linked into the product.  Build
+ * with -ffp-contract=off so every double operation rounds once, in the order
+ * the reference source writes it.  Pins (tests/test_oracle_*.py):
+ *   - oracle/_ref (the reference header compiled for the host): bit-equal
+ *     matrices, validity, limits and log scales on random trajectories;
+ *   - the published zeta-limit and random-limit f64 records;
+ *   - exact rational walks (Fraction) for short N: ratios, validity,
+ *     log_scale + log|q| = log|q_exact|;
+ *   - RNS residues against exact rational entries mod prime moduli, the
+ *     one-step solve identities on composite moduli, CRT against Python ints.
+ */
+#include "cmf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EPS 2.220446049250313e-16
+
+/* make_cmf_desc, cmf_pfq.cuh:50-61 */
+orc_desc orc_make_desc(int p, int q, double z) {
+    orc_desc d;
+    d.p = p;
+    d.q = q;
+    d.z = z;
+    d.naxes = p + q;
+    d.rank = (p > q + 1) ? p : q + 1;
+    if (z == 1.0 && p == q + 1) d.rank -= 1;
+    return d;
+}
+
+/* llround with the documented out-of-range policy (see header). */
+int64_t orc_llround(double x) {
+    if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) return INT64_MIN;
+    double r = round(x); /* half away from zero, exact for all doubles */
+    return (int64_t)r;
+}
+
+/* to_mod, cmf_walk_rns.cu:61-66 */
+uint32_t orc_to_mod(double x, uint32_t m) {
+    int64_t v = orc_llround(x);
+    if (v >= 0) return (uint32_t)((uint64_t)v % m);
+    uint64_t neg = (uint64_t)0 - (uint64_t)v;
+    return (uint32_t)((m - (neg % m)) % m);
+}
+
+/* ---------------------------------------------------------------- f64 -- */
+
+/* normalize_projective, cmf_pfq.cuh:100-116 */
+static int normalize(double *M, int dim, double *log_scale) {
+    double mx = 0.0;
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) {
+            double v = fabs(M[r * 8 + c]);
+            if (v > mx) mx = v;
+        }
+    if (!(mx > 0.0) || !isfinite(mx)) return 0;
+    double inv = 1.0 / mx;
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) M[r * 8 + c] *= inv;
+    *log_scale += log(mx);
+    return 1;
+}
+
+/* poly_theta_plus, cmf_pfq.cuh:127-141: coefficients of prod_j (theta + v_j),
+ * ascending.  Each new coefficient is 0 + c[k]*v, then + c[k-1]-shifted, in
+ * exactly the reference's accumulation order. */
+void orc_poly_theta_plus(const double *vals, int count, double *out) {
+    for (int k = 0; k <= count; ++k) out[k] = 0.0;
+    out[0] = 1.0;
+    for (int j = 0; j < count; ++j) {
+        double acc[ORC_MAX_DIM + 2];
+        for (int k = 0; k < ORC_MAX_DIM + 2; ++k) acc[k] = 0.0;
+        for (int k = 0; k <= j; ++k) {
+            double prod = out[k] * vals[j];
+            acc[k] = acc[k] + prod;
+            acc[k + 1] = acc[k + 1] + out[k];
+        }
+        for (int k = 0; k <= j + 1; ++k) out[k] = acc[k];
+    }
+}
+
+/* theta_companion_column, cmf_pfq.cuh:146-181 */
+int orc_companion_column(const double *pos, double z, int dim, int p, int q, double *col) {
+    double xv[ORC_MAX_AXES] = {0}, yv[ORC_MAX_AXES] = {0};
+    for (int i = 0; i < p; ++i) xv[i] = pos[i];
+    for (int i = 0; i < q; ++i) yv[i] = pos[p + i] - 1.0;
+    double px[ORC_MAX_DIM + 2] = {0}, py[ORC_MAX_DIM + 2] = {0};
+    orc_poly_theta_plus(xv, p, px);
+    orc_poly_theta_plus(yv, q, py);
+    double D[ORC_MAX_DIM + 2] = {0};
+    for (int k = 0; k <= p; ++k) D[k] = -z * px[k];
+    for (int k = 0; k <= q; ++k) D[k + 1] = D[k + 1] + py[k];
+    double lead = D[dim];
+    double lscale = 1.0;
+    if (dim == p && p == q + 1) {
+        lscale = fmax(1.0, fabs(1.0 - z));
+    } else {
+        for (int i = 0; i < p; ++i) lscale += fabs(pos[i]);
+        for (int i = 0; i < q; ++i) lscale += fabs(pos[p + i] - 1.0);
+    }
+    if (!isfinite(lead) || fabs(lead) <= 64.0 * EPS * lscale) return 0;
+    for (int i = 0; i < dim; ++i) {
+        col[i] = -D[i] / lead;
+        if (!isfinite(col[i])) return 0;
+    }
+    return 1;
+}
+
+/* right_mul_affine_companion, cmf_pfq.cuh:186-239.  M is 8x8 row-major. */
+int orc_right_mul(double *M, int dim, const double *col, double denom, int inverse,
+                  double *log_scale) {
+    if (!isfinite(denom) || fabs(denom) <= 64.0 * EPS) return 0;
+    double out[64];
+    double ia = 1.0 / denom;
+    if (!inverse) {
+        for (int r = 0; r < dim; ++r) {
+            const double *row = M + r * 8;
+            double dot = 0.0;
+            for (int k = 0; k < dim; ++k) dot = dot + row[k] * col[k];
+            for (int j = 0; j + 1 < dim; ++j) out[r * 8 + j] = row[j] + row[j + 1] * ia;
+            out[r * 8 + dim - 1] = row[dim - 1] + dot * ia;
+        }
+    } else {
+        for (int r = 0; r < dim; ++r) {
+            const double *row = M + r * 8;
+            double al[ORC_MAX_DIM] = {0}, be[ORC_MAX_DIM] = {0};
+            al[dim - 1] = 0.0;
+            be[dim - 1] = 1.0;
+            for (int j = dim - 2; j >= 0; --j) {
+                al[j] = row[j] - al[j + 1] * ia;
+                be[j] = -be[j + 1] * ia;
+            }
+            double da = 0.0, db = 0.0;
+            for (int k = 0; k < dim; ++k) {
+                da = da + al[k] * col[k];
+                db = db + be[k] * col[k];
+            }
+            double sd = 1.0 + db * ia;
+            double ss = fmax(1.0, fabs(db * ia));
+            if (!isfinite(sd) || fabs(sd) <= 128.0 * EPS * ss) return 0;
+            double t = (row[dim - 1] - da * ia) / sd;
+            if (!isfinite(t)) return 0;
+            for (int j = 0; j < dim; ++j) {
+                out[r * 8 + j] = al[j] + be[j] * t;
+                if (!isfinite(out[r * 8 + j])) return 0;
+            }
+        }
+    }
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) M[r * 8 + c] = out[r * 8 + c];
+    return normalize(M, dim, log_scale);
+}
+
+/* apply_axis_step, cmf_pfq.cuh:255-291 */
+int orc_axis_step(double *M, double *pos, int axis, int sign, const orc_desc *d,
+                  double *log_scale) {
+    double ev[ORC_MAX_AXES];
+    memcpy(ev, pos, sizeof(ev));
+    double denom;
+    int inverse = 0;
+    if (axis < d->p) {
+        if (sign > 0) {
+            denom = pos[axis];
+        } else {
+            ev[axis] -= 1.0;
+            denom = pos[axis] - 1.0;
+            inverse = 1;
+        }
+    } else {
+        if (sign < 0) {
+            denom = pos[axis] - 1.0;
+        } else {
+            ev[axis] += 1.0;
+            denom = pos[axis];
+            inverse = 1;
+        }
+    }
+    double col[ORC_MAX_DIM] = {0};
+    if (!orc_companion_column(ev, d->z, d->rank, d->p, d->q, col)) return 0;
+    if (!orc_right_mul(M, d->rank, col, denom, inverse, log_scale)) return 0;
+    pos[axis] += (double)sign;
+    return 1;
+}
+
+/* apply_trajectory_step, cmf_pfq.cuh:305-322: largest |dir| level first,
+ * axes from the last to the first within a level. */
+static int traj_step(double *M, double *pos, const int32_t *dir, const orc_desc *d,
+                     double *log_scale) {
+    int top = 0;
+    for (int a = 0; a < d->naxes; ++a) {
+        int v = abs(dir[a]);
+        if (v > top) top = v;
+    }
+    for (int lev = top; lev >= 1; --lev)
+        for (int a = d->naxes - 1; a >= 0; --a) {
+            if (abs(dir[a]) < lev) continue;
+            if (!orc_axis_step(M, pos, a, dir[a] > 0 ? 1 : -1, d, log_scale)) return 0;
+        }
+    return 1;
+}
+
+static void identity8(double *M, int dim) {
+    for (int r = 0; r < 8; ++r)
+        for (int c = 0; c < 8; ++c) M[r * 8 + c] = (r == c && r < dim) ? 1.0 : 0.0;
+}
+
+/* forward_walk, cmf_pfq.cuh:348-371 */
+int orc_forward_walk(const int32_t *dir, const double *start, const orc_desc *d, int N, int h,
+                     orc_snaps *s) {
+    memset(s, 0, sizeof(*s));
+    s->dim = d->rank;
+    if (N <= 0 || h <= 0 || N - 2 * h <= 0) return 0;
+    double M[64], pos[ORC_MAX_AXES];
+    identity8(M, d->rank);
+    memcpy(pos, start, sizeof(pos));
+    int cp1 = N - 2 * h, cp2 = N - h;
+    double ls = 0.0;
+    for (int step = 1; step <= N; ++step) {
+        if (!traj_step(M, pos, dir, d, &ls)) return 0;
+        int slot = step == cp1 ? 0 : step == cp2 ? 1 : step == N ? 2 : -1;
+        if (slot >= 0) {
+            memcpy(s->m[slot], M, sizeof(M));
+            s->ls[slot] = ls;
+        }
+    }
+    s->valid = 1;
+    return 1;
+}
+
+#define SN(s, k, r, c) ((s)->m[k][r][c])
+
+/* compute_limits, cmf_pfq.cuh:385-460, reduced to what the drivers read:
+ * the candidate count (capped at max) and the first Aitken candidate, else
+ * candidate 0 (selection rule of cmf_walk_f64.cu:128-140; method 2 is never
+ * produced).  Candidates are counted in the reference's order. */
+void orc_limits_summary(const orc_snaps *s, int *n_limits, double *limit) {
+    const int dim = s->dim;
+    const double DEN_MIN = 1e-280;
+    int n = 0;
+    double first = NAN, first_aitken = NAN;
+    const int cap = ORC_MAX_LIMITS;
+#define PUSH(L, isA)                                               \
+    do {                                                           \
+        if (n < cap) {                                             \
+            if (n == 0) first = (L);                               \
+            if ((isA) && isnan(first_aitken)) first_aitken = (L);  \
+            ++n;                                                   \
+        }                                                          \
+    } while (0)
+    {
+        int c = dim - 1;
+        double p3 = SN(s, 2, 0, c), q3 = SN(s, 2, 1, c);
+        double p2 = SN(s, 1, 0, c), q2 = SN(s, 1, 1, c);
+        double p1 = SN(s, 0, 0, c), q1 = SN(s, 0, 1, c);
+        if (fabs(q3) > DEN_MIN && fabs(q2) > DEN_MIN && fabs(q1) > DEN_MIN) {
+            double r1 = p1 / q1, r2 = p2 / q2, r3 = p3 / q3;
+            if (isfinite(r1) && isfinite(r2) && isfinite(r3)) {
+                double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+                double den = r1 - 2.0 * r2 + r3;
+                if (fabs(den) > 1e-14 * sc) {
+                    double dd = r2 - r3;
+                    double L = r3 - dd * dd / den;
+                    if (isfinite(L)) PUSH(L, 1);
+                }
+                if (fabs(r3) > 1e-15) PUSH(r3, 0);
+            }
+        }
+    }
+    for (int c = 0; c < dim; ++c)
+        for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j) {
+                if (i == j) continue;
+                if (c == dim - 1 && i == 0 && j == 1) continue;
+                double d1 = SN(s, 0, j, c), d2 = SN(s, 1, j, c), d3 = SN(s, 2, j, c);
+                if (fabs(d1) < DEN_MIN || fabs(d2) < DEN_MIN || fabs(d3) < DEN_MIN) continue;
+                double r1 = SN(s, 0, i, c) / d1;
+                double r2 = SN(s, 1, i, c) / d2;
+                double r3 = SN(s, 2, i, c) / d3;
+                if (!isfinite(r1) || !isfinite(r2) || !isfinite(r3)) continue;
+                double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+                double den = r1 - 2.0 * r2 + r3;
+                if (fabs(den) > 1e-14 * sc) {
+                    double dd = r2 - r3;
+                    double L = r3 - dd * dd / den;
+                    if (isfinite(L) && fabs(L) > 1e-15) PUSH(L, 1);
+                }
+                if (fabs(r3) > 1e-15) PUSH(r3, 0);
+            }
+#undef PUSH
+    *n_limits = n;
+    *limit = !isnan(first_aitken) ? first_aitken : (n > 0 ? first : NAN);
+}
+
+/* compute_delta, cmf_pfq.cuh:476-527 */
+double orc_delta(const orc_snaps *s, double L) {
+    if (!s->valid) return NAN;
+    const int c = s->dim - 1;
+    double pa = SN(s, 1, 0, c), qa = SN(s, 1, 1, c);
+    if (fabs(qa) < 1e-280) return NAN;
+    double log_q = log(fabs(qa)) + s->ls[1];
+    if (log_q <= 0.0) return NAN;
+    double pN = SN(s, 2, 0, c), qN = SN(s, 2, 1, c);
+    double LN = (fabs(qN) > 1e-280) ? pN / qN : NAN;
+    double Lu = L;
+    if (!isfinite(Lu)) {
+        double p2 = SN(s, 1, 0, c), q2 = SN(s, 1, 1, c);
+        double p1 = SN(s, 0, 0, c), q1 = SN(s, 0, 1, c);
+        if (fabs(q2) > 1e-280 && fabs(q1) > 1e-280) {
+            double r1 = p1 / q1, r2 = p2 / q2, r3 = LN;
+            double den = r1 - 2.0 * r2 + r3;
+            if (isfinite(den) && fabs(den) > 1e-30) {
+                double dd = r2 - r3;
+                Lu = r3 - dd * dd / den;
+            }
+        }
+        if (!isfinite(Lu)) Lu = LN;
+    }
+    if (!isfinite(Lu)) return NAN;
+    double approx = pa / qa;
+    if (!isfinite(approx)) return NAN;
+    double diff = fabs(Lu - approx);
+    if (diff <= 0.0) return INFINITY;
+    if (!isfinite(diff)) return NAN;
+    return -(1.0 + log(diff) / log_q);
+}
+
+/* compute_precision, cmf_pfq.cuh:531-552 */
+double orc_precision(const orc_snaps *s) {
+    if (!s->valid) return NAN;
+    const int c = s->dim - 1;
+    double pN = SN(s, 2, 0, c), qN = SN(s, 2, 1, c);
+    double p2 = SN(s, 1, 0, c), q2 = SN(s, 1, 1, c);
+    if (fabs(qN) < 1e-280 || fabs(q2) < 1e-280) return NAN;
+    double a = pN / qN, b = p2 / q2;
+    if (!isfinite(a) || !isfinite(b)) return NAN;
+    double diff = fabs(a - b);
+    if (diff <= 0.0) return INFINITY;
+    return -log10(diff);
+}
+
+/* compute_kamidelta, cmf_pfq.cuh:563-596 */
+double orc_kamidelta(const orc_snaps *s, double Lt) {
+    if (!s->valid) return NAN;
+    const int c = s->dim - 1;
+    double pN = SN(s, 2, 0, c), qN = SN(s, 2, 1, c);
+    double p2 = SN(s, 1, 0, c), q2 = SN(s, 1, 1, c);
+    double p1 = SN(s, 0, 0, c), q1 = SN(s, 0, 1, c);
+    if (fabs(qN) < 1e-280 || fabs(q2) < 1e-280 || fabs(q1) < 1e-280) return NAN;
+    double LN = pN / qN, L2 = p2 / q2, L1 = p1 / q1;
+    if (!isfinite(LN) || !isfinite(L2) || !isfinite(L1)) return NAN;
+    double La = NAN;
+    double den = L1 - 2.0 * L2 + LN;
+    if (fabs(den) > 1e-30) {
+        double dd = L2 - LN;
+        La = LN - dd * dd / den;
+    }
+    double L = isfinite(Lt) ? Lt : (isfinite(La) ? La : LN);
+    return orc_delta(s, L);
+}
+
+/* Per-trajectory result, walk_kernel_f64 body (cmf_walk_f64.cu:106-153) and
+ * the RNS kernel's tail (cmf_walk_rns.cu:300-319). */
+void orc_finish(const orc_snaps *s, int ok, orc_result *r) {
+    r->valid = ok ? 1 : 0;
+    if (!ok) {
+        r->limit = r->delta = r->p = r->q = r->precision = r->log_scale = NAN;
+        r->n_limits = 0;
+        return;
+    }
+    int nl;
+    double L;
+    orc_limits_summary(s, &nl, &L);
+    r->n_limits = nl;
+    r->limit = L;
+    const int c = s->dim - 1;
+    r->p = SN(s, 2, 0, c);
+    r->q = SN(s, 2, 1, c);
+    r->log_scale = s->ls[2];
+    r->precision = orc_precision(s);
+    r->delta = orc_kamidelta(s, r->limit);
+}
+
+void orc_walk_f64_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
+                        int N, int h, orc_result *res, orc_snaps *snaps_out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int i = 0; i < n; ++i) {
+        orc_snaps s;
+        int ok = orc_forward_walk(dirs + (size_t)i * 16, starts + (size_t)i * 16, d, N, h, &s);
+        orc_finish(&s, ok, &res[i]);
+        if (snaps_out) snaps_out[i] = s;
+    }
+}
+
+/* ---------------------------------------------------------------- RNS -- */
+
+/* Modulus table, cmf_walk_rns.cu:26-43 (data; 53 of the 64 are composite). */
+static const uint32_t MODULI[ORC_MAX_PRIMES] = {
+    1073741789u, 1073741783u, 1073741741u, 1073741723u, 1073741717u, 1073741707u,
+    1073741693u, 1073741683u, 1073741669u, 1073741657u, 1073741641u, 1073741623u,
+    1073741611u, 1073741597u, 1073741587u, 1073741573u, 1073741563u, 1073741551u,
+    1073741537u, 1073741527u, 1073741513u, 1073741503u, 1073741491u, 1073741477u,
+    1073741467u, 1073741453u, 1073741441u, 1073741427u, 1073741413u, 1073741403u,
+    1073741391u, 1073741377u, 1073741363u, 1073741353u, 1073741341u, 1073741329u,
+    1073741317u, 1073741307u, 1073741293u, 1073741281u, 1073741267u, 1073741257u,
+    1073741243u, 1073741231u, 1073741219u, 1073741207u, 1073741193u, 1073741183u,
+    1073741171u, 1073741157u, 1073741147u, 1073741137u, 1073741123u, 1073741111u,
+    1073741097u, 1073741087u, 1073741077u, 1073741063u, 1073741051u, 1073741039u,
+    1073741027u, 1073741017u, 1073741003u, 1073740991u};
+
+uint32_t orc_modulus(int k) { return (k >= 0 && k < ORC_MAX_PRIMES) ? MODULI[k] : 0u; }
+
+static inline uint32_t mulm(uint32_t a, uint32_t b, uint32_t m) {
+    return (uint32_t)(((uint64_t)a * b) % m);
+}
+static inline uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
+    uint64_t r = (uint64_t)a + b;
+    return (uint32_t)(r >= m ? r - m : r);
+}
+static inline uint32_t subm(uint32_t a, uint32_t b, uint32_t m) {
+    return a >= b ? a - b : (uint32_t)((uint64_t)a + m - b);
+}
+
+/* a^e mod m by binary exponentiation (mod_inv, cmf_walk_rns.cu:68-78, uses
+ * e = m-2: a Fermat inverse for prime m, a fixed power otherwise). */
+uint32_t orc_mod_pow(uint32_t a, uint32_t e, uint32_t m) {
+    uint64_t r = 1 % m, b = a % m;
+    while (e) {
+        if (e & 1u) r = (r * b) % m;
+        b = (b * b) % m;
+        e >>= 1;
+    }
+    return (uint32_t)r;
+}
+
+/* right_mul_rns on one lane, cmf_walk_rns.cu:139-179 (M is 8x8). */
+int orc_rns_right_mul_lane(uint32_t *M, int dim, const uint32_t *col, uint32_t denom,
+                           int inverse, uint32_t m) {
+    uint32_t out[64];
+    uint32_t ia = orc_mod_pow(denom, m - 2, m);
+    if (!inverse) {
+        for (int r = 0; r < dim; ++r) {
+            const uint32_t *row = M + r * 8;
+            uint32_t dot = 0;
+            for (int k = 0; k < dim; ++k) dot = addm(dot, mulm(row[k], col[k], m), m);
+            for (int j = 0; j + 1 < dim; ++j) out[r * 8 + j] = addm(row[j], mulm(row[j + 1], ia, m), m);
+            out[r * 8 + dim - 1] = addm(row[dim - 1], mulm(dot, ia, m), m);
+        }
+    } else {
+        for (int r = 0; r < dim; ++r) {
+            const uint32_t *row = M + r * 8;
+            uint32_t al[ORC_MAX_DIM] = {0}, be[ORC_MAX_DIM] = {0};
+            al[dim - 1] = 0;
+            be[dim - 1] = 1;
+            for (int j = dim - 2; j >= 0; --j) {
+                al[j] = subm(row[j], mulm(al[j + 1], ia, m), m);
+                be[j] = subm(0, mulm(be[j + 1], ia, m), m);
+            }
+            uint32_t da = 0, db = 0;
+            for (int k = 0; k < dim; ++k) {
+                da = addm(da, mulm(al[k], col[k], m), m);
+                db = addm(db, mulm(be[k], col[k], m), m);
+            }
+            uint32_t sd = addm(1, mulm(db, ia, m), m);
+            if (sd == 0) return 0;
+            uint32_t isd = orc_mod_pow(sd, m - 2, m);
+            uint32_t t = mulm(subm(row[dim - 1], mulm(da, ia, m), m), isd, m);
+            for (int j = 0; j < dim; ++j) out[r * 8 + j] = addm(al[j], mulm(be[j], t, m), m);
+        }
+    }
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) M[r * 8 + c] = out[r * 8 + c];
+    return 1;
+}
+
+/* apply_axis_step_rns + theta_companion_column_rns, cmf_walk_rns.cu:101-128,
+ * 185-205: the column is the f64 column, rounded per lane by to_mod. */
+static int rns_axis_step(uint32_t *W /*[np][64]*/, double *pos, int axis, int sign,
+                         const orc_desc *d, int np) {
+    double ev[ORC_MAX_AXES];
+    memcpy(ev, pos, sizeof(ev));
+    double denom;
+    int inverse;
+    if (axis < d->p) {
+        inverse = sign < 0;
+        if (sign > 0) {
+            denom = pos[axis];
+        } else {
+            ev[axis] -= 1.0;
+            denom = pos[axis] - 1.0;
+        }
+    } else {
+        inverse = sign > 0;
+        if (sign < 0) {
+            denom = pos[axis] - 1.0;
+        } else {
+            ev[axis] += 1.0;
+            denom = pos[axis];
+        }
+    }
+    double col[ORC_MAX_DIM] = {0};
+    if (!orc_companion_column(ev, d->z, d->rank, d->p, d->q, col)) return 0;
+    if (!isfinite(denom) || fabs(denom) <= 64.0 * EPS) return 0;
+    for (int k = 0; k < np; ++k) {
+        uint32_t m = MODULI[k];
+        uint32_t cr[ORC_MAX_DIM];
+        for (int i = 0; i < d->rank; ++i) cr[i] = orc_to_mod(col[i], m);
+        uint32_t dr = orc_to_mod(denom, m);
+        if (!orc_rns_right_mul_lane(W + k * 64, d->rank, cr, dr, inverse, m)) return 0;
+    }
+    pos[axis] += (double)sign;
+    return 1;
+}
+
+static int rns_traj_step(uint32_t *W, double *pos, const int32_t *dir, const orc_desc *d,
+                         int np) {
+    int top = 0;
+    for (int a = 0; a < d->naxes; ++a) {
+        int v = abs(dir[a]);
+        if (v > top) top = v;
+    }
+    for (int lev = top; lev >= 1; --lev)
+        for (int a = d->naxes - 1; a >= 0; --a) {
+            if (abs(dir[a]) < lev) continue;
+            if (!rns_axis_step(W, pos, a, dir[a] > 0 ? 1 : -1, d, np)) return 0;
+        }
+    return 1;
+}
+
+/* One entry of rns_to_double's Garner CRT (cmf_walk_rns.cu:232-247), with
+ * the reference's uint64 wrap-around kept. */
+double orc_crt_entry(uint32_t r0, uint32_t r1, int np, uint64_t *val_out) {
+    if (np < 2) {
+        if (val_out) *val_out = r0;
+        return (double)r0;
+    }
+    uint64_t m0 = MODULI[0], m1 = MODULI[1];
+    uint64_t diff = (r1 >= r0) ? (uint64_t)(r1 - r0) : ((uint64_t)r1 + m1 - r0);
+    uint64_t inv = orc_mod_pow((uint32_t)(m0 % m1), (uint32_t)(m1 - 2), (uint32_t)m1);
+    uint64_t t = (diff * inv) % m1;
+    uint64_t v = (uint64_t)r0 + m0 * t;
+    uint64_t half = (m0 * m1) / 2;
+    if (val_out) *val_out = v;
+    return (v > half) ? -(double)(m0 * m1 - v) : (double)v;
+}
+
+static void rns_identity(uint32_t *W, int np, int dim) {
+    memset(W, 0, sizeof(uint32_t) * 64 * np);
+    for (int k = 0; k < np; ++k)
+        for (int i = 0; i < dim; ++i) W[k * 64 + i * 8 + i] = 1;
+}
+
+/* walk_kernel_rns, cmf_walk_rns.cu:260-319 */
+int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int N, int h,
+                 int np, orc_result *r, orc_snaps *s, uint32_t *resid_out) {
+    uint32_t W[ORC_MAX_PRIMES * 64];
+    const int dim = d->rank;
+    rns_identity(W, np, dim);
+    double pos[ORC_MAX_AXES];
+    memcpy(pos, start, sizeof(pos));
+    memset(s, 0, sizeof(*s));
+    s->dim = dim;
+    int cp1 = N - 2 * h, cp2 = N - h;
+    int ok = 1;
+    double ls = 0.0;
+    for (int step = 1; step <= N; ++step) {
+        if (!rns_traj_step(W, pos, dir, d, np)) {
+            ok = 0;
+            break;
+        }
+        if (step % 100 == 0 || step == cp1 || step == cp2 || step == N) {
+            double wm[64];
+            memset(wm, 0, sizeof(wm));
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j)
+                    wm[i * 8 + j] = orc_crt_entry(W[i * 8 + j], np >= 2 ? W[64 + i * 8 + j] : 0,
+                                                  np, NULL);
+            (void)normalize(wm, dim, &ls);
+            if (step == cp1) { memcpy(s->m[0], wm, sizeof(wm)); s->ls[0] = ls; }
+            if (step == cp2) { memcpy(s->m[1], wm, sizeof(wm)); s->ls[1] = ls; }
+            if (step == N) { memcpy(s->m[2], wm, sizeof(wm)); s->ls[2] = ls; }
+            for (int k = 0; k < np; ++k)
+                for (int i = 0; i < dim; ++i)
+                    for (int j = 0; j < dim; ++j)
+                        W[k * 64 + i * 8 + j] = orc_to_mod(wm[i * 8 + j], MODULI[k]);
+        }
+    }
+    if (resid_out) memcpy(resid_out, W, sizeof(uint32_t) * 64 * np);
+    s->valid = ok;
+    orc_finish(s, ok, r);
+    return ok;
+}
+
+void orc_walk_rns_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
+                        int N, int h, int np, orc_result *res, orc_snaps *snaps_out,
+                        int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int i = 0; i < n; ++i) {
+        orc_snaps s;
+        orc_walk_rns(dirs + (size_t)i * 16, starts + (size_t)i * 16, d, N, h, np, &res[i], &s,
+                     NULL);
+        if (snaps_out) snaps_out[i] = s;
+    }
+}
+
+int orc_rns_raw_walk(const int32_t *dir, const double *start, const orc_desc *d, int steps,
+                     int np, uint32_t *resid_out) {
+    uint32_t W[ORC_MAX_PRIMES * 64];
+    rns_identity(W, np, d->rank);
+    double pos[ORC_MAX_AXES];
+    memcpy(pos, start, sizeof(pos));
+    int ok = 1;
+    for (int step = 1; step <= steps; ++step)
+        if (!rns_traj_step(W, pos, dir, d, np)) {
+            ok = 0;
+            break;
+        }
+    memcpy(resid_out, W, sizeof(uint32_t) * 64 * np);
+    return ok;
+}
+
+/* ------------------------------------------------------ legacy inputs -- */
+
+/* std::mt19937_64 (the C++ standard's 64-bit Mersenne twister). */
+void orc_mt64_seed(orc_mt64 *g, uint64_t seed) {
+    g->mt[0] = seed;
+    for (int i = 1; i < 312; ++i)
+        g->mt[i] = 6364136223846793005ULL * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+    g->idx = 312;
+}
+
+uint64_t orc_mt64_next(orc_mt64 *g) {
+    const uint64_t UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL;
+    if (g->idx >= 312) {
+        for (int i = 0; i < 312; ++i) {
+            uint64_t x = (g->mt[i] & UM) | (g->mt[(i + 1) % 312] & LM);
+            uint64_t xa = x >> 1;
+            if (x & 1ULL) xa ^= 0xB5026F5AA96619E9ULL;
+            g->mt[i] = g->mt[(i + 156) % 312] ^ xa;
+        }
+        g->idx = 0;
+    }
+    uint64_t y = g->mt[g->idx++];
+    y ^= (y >> 29) & 0x5555555555555555ULL;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+    y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+    y ^= y >> 43;
+    return y;
+}
+
+/* Effective inputs of the legacy drivers (cmf_walk_f64.cu:166-199, the
+ * generators at :61-91).  The reference writes 128-byte Position records and
+ * 16-int direction records into an array of 192-byte TrajectoryInput
+ * structs, so the written bytes land in neighbouring structs.  We rebuild
+ * the byte image exactly and read it back as structs. */
+typedef struct { int32_t dir[16]; double v[16]; } legacy_rec; /* 192 bytes */
+
+void orc_legacy_inputs(int p, int q, int n, int mode, int use_zeta, int32_t *dirs,
+                       double *starts) {
+    const int nax = p + q;
+    legacy_rec *img = (legacy_rec *)calloc((size_t)(n > 0 ? n : 1), sizeof(legacy_rec));
+    if (mode == 0) {
+        if (use_zeta) {
+            for (int i = 0; i < n; ++i) {
+                for (int a = 0; a < 16; ++a) img[i].v[a] = 0.0; /* Position pos; uninit tail */
+                for (int a = 0; a < p; ++a) img[i].v[a] = 1.0;
+                for (int a = 0; a < q; ++a) img[i].v[p + a] = 2.0;
+            }
+        } else {
+            /* points written at 128-byte stride (cmf_walk_f64.cu:178) */
+            double *pts = (double *)img;
+            orc_mt64 g;
+            orc_mt64_seed(&g, 42);
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < nax; ++a)
+                    pts[(size_t)i * 16 + a] = (double)(2 + (int)(orc_mt64_next(&g) % 19));
+        }
+        for (int i = 0; i < n; ++i) {
+            memset(img[i].dir, 0, sizeof(img[i].dir));
+            img[i].dir[0] = 1;
+        }
+    } else {
+        double fs[16];
+        for (int a = 0; a < 16; ++a) fs[a] = 0.0;
+        for (int a = 0; a < nax; ++a)
+            fs[a] = use_zeta ? (a < p ? 1.0 : 2.0) : (a < p ? 2.0 : 3.0);
+        for (int i = 0; i < n; ++i) memcpy(img[i].v, fs, sizeof(fs));
+        /* directions written at 16-int stride (cmf_walk_f64.cu:198) */
+        int32_t *ints = (int32_t *)img;
+        orc_mt64 g;
+        orc_mt64_seed(&g, 42);
+        const int dmax = 3;
+        for (int i = 0; i < n; ++i) {
+            for (int a = 0; a < nax; ++a)
+                ints[(size_t)i * 16 + a] = (int32_t)(orc_mt64_next(&g) % (2 * dmax + 1)) - dmax;
+            int zero = 1;
+            for (int a = 0; a < nax; ++a)
+                if (ints[(size_t)i * 16 + a] != 0) { zero = 0; break; }
+            if (zero) ints[(size_t)i * 16] = 1;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        memcpy(dirs + (size_t)i * 16, img[i].dir, 16 * sizeof(int32_t));
+        memcpy(starts + (size_t)i * 16, img[i].v, 16 * sizeof(double));
+    }
+    free(img);
+}

@@ -1,0 +1,116 @@
+/*
+ * cmf_oracle.h -- CPU oracle for the pFq CMF walk (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is the checker, never the product: only tests/, the smoke check and
+ * the cpu_baseline leg of bench.py may load it.  It restates, operation by
+ * operation and with -ffp-contract=off, the recurrence of
+ *   cuda/cmf_pfq.cuh        (f64 walk, limits, delta, precision, kamidelta)
+ *   cuda/cmf_walk_f64.cu    (per-trajectory result selection, legacy inputs)
+ *   cuda/cmf_walk_rns.cu    (RNS walk, 2-prime CRT, 100-step re-encode)
+ * Each function names the reference lines it follows in its definition.
+ *
+ * Policies the reference leaves open (shared with the GPU kernels):
+ *   - llround of |x| >= 2^63 or NaN returns INT64_MIN (x86 cvttsd2si
+ *     behaviour); the unsigned negation of INT64_MIN is 2^63.
+ *   - RNS/df64 snapshots that are never taken (N-2h <= 0) are zero matrices.
+ *   - RNS p/q of an invalid trajectory are NaN.
+ */
+#ifndef CMF_ORACLE_H
+#define CMF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_DIM 8
+#define ORC_MAX_AXES 16
+#define ORC_MAX_PRIMES 64
+#define ORC_MAX_LIMITS 256
+
+typedef struct orc_desc {
+    int p, q, rank, naxes;
+    double z;
+} orc_desc;
+
+/* One trajectory's outputs, same fields as TrajectoryResult
+ * (cmf_walk_f64.cu:41-51) / RNSResult (cmf_walk_rns.cu:90-99). */
+typedef struct orc_result {
+    double limit, delta, p, q, precision, log_scale;
+    int32_t valid, n_limits;
+} orc_result;
+
+/* Snapshot triple: matrices after N-2h, N-h, N (row-major 8x8, only the
+ * leading rank x rank block is meaningful) and their log scales. */
+typedef struct orc_snaps {
+    double m[3][ORC_MAX_DIM][ORC_MAX_DIM];
+    double ls[3];
+    int dim;
+    int valid;
+} orc_snaps;
+
+orc_desc orc_make_desc(int p, int q, double z);
+int64_t orc_llround(double x);
+uint32_t orc_to_mod(double x, uint32_t m);
+uint32_t orc_modulus(int k);
+
+/* single pieces, exported so tests can pin them one by one */
+void orc_poly_theta_plus(const double *vals, int count, double *out);
+int orc_companion_column(const double *pos, double z, int dim, int p, int q, double *col);
+int orc_right_mul(double *M /*8x8*/, int dim, const double *col, double denom,
+                  int inverse, double *log_scale);
+int orc_axis_step(double *M, double *pos, int axis, int sign, const orc_desc *d,
+                  double *log_scale);
+
+/* f64 walk of one trajectory; returns snaps.valid */
+int orc_forward_walk(const int32_t *dir, const double *start, const orc_desc *d,
+                     int N, int h, orc_snaps *s);
+void orc_limits_summary(const orc_snaps *s, int *n_limits, double *limit);
+double orc_precision(const orc_snaps *s);
+double orc_kamidelta(const orc_snaps *s, double L);
+double orc_delta(const orc_snaps *s, double L);
+void orc_finish(const orc_snaps *s, int ok, orc_result *r);
+
+/* Batch f64 walk: dirs/starts trajectory-major [n][16]; OpenMP over
+ * trajectories with nthreads (<=0: all). snaps_out may be NULL. */
+void orc_walk_f64_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
+                        int N, int h, orc_result *res, orc_snaps *snaps_out, int nthreads);
+
+/* RNS walk of one trajectory (cmf_walk_rns.cu:260-320).  If resid_out is
+ * non-NULL it receives the final residues [n_primes][8][8] (before any
+ * re-encode at step N is applied it is the re-encoded state, as the
+ * reference leaves it).  crt_out, if non-NULL, receives the raw CRT value
+ * (uint64) of every entry at the final snapshot [8][8]. */
+int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int N, int h,
+                 int n_primes, orc_result *r, orc_snaps *s, uint32_t *resid_out);
+void orc_walk_rns_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
+                        int N, int h, int n_primes, orc_result *res, orc_snaps *snaps_out,
+                        int nthreads);
+/* RNS walk of 'steps' trajectory steps with NO CRT/re-encode (steps < 100
+ * reproduces the reference exactly); residues out [n_primes][8][8]. */
+int orc_rns_raw_walk(const int32_t *dir, const double *start, const orc_desc *d, int steps,
+                     int n_primes, uint32_t *resid_out);
+/* One RNS axis step on lane k with explicit column residues (for invariants) */
+int orc_rns_right_mul_lane(uint32_t *M /*8x8*/, int dim, const uint32_t *col, uint32_t denom,
+                           int inverse, uint32_t m);
+uint32_t orc_mod_pow(uint32_t a, uint32_t e, uint32_t m);
+/* 2-prime Garner CRT of one entry as the reference computes it
+ * (cmf_walk_rns.cu:232-244); *val_out gets the uint64 value v. */
+double orc_crt_entry(uint32_t r0, uint32_t r1, int n_primes, uint64_t *val_out);
+
+/* std::mt19937_64 */
+typedef struct orc_mt64 { uint64_t mt[312]; int idx; } orc_mt64;
+void orc_mt64_seed(orc_mt64 *g, uint64_t seed);
+uint64_t orc_mt64_next(orc_mt64 *g);
+
+/* Legacy drivers' effective inputs, including the stride aliasing of
+ * cmf_walk_f64.cu:178 and :198.  mode 0 = limit, 1 = delta.
+ * Output trajectory-major dirs[n][16], starts[n][16]. */
+void orc_legacy_inputs(int p, int q, int n, int mode, int use_zeta, int32_t *dirs,
+                       double *starts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,364 @@
+"""CPU oracle for the pFq CMF walk -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module; the product (the HIP walk engine behind the C ABI) never
+does.  It wraps:
+
+* ``liborc.so`` -- the C restatement in cmf_oracle.c of the reference's
+  cuda/cmf_pfq.cuh, cuda/cmf_walk_f64.cu and cuda/cmf_walk_rns.cu;
+* ``_ref/libcmfref.so`` -- the reference's own cmf_pfq.cuh compiled for the
+  host (present only where the reference tree was available at build time);
+* an exact rational walk (``fractions.Fraction``) used to pin the restatement
+  on short walks: the axis matrices are built densely from their definition
+  (I + C/a with the theta-companion C), independently of the O(d^2) update.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcmfref.so")
+
+MAX_AXES = 16
+MAX_DIM = 8
+
+
+class Desc(C.Structure):
+    _fields_ = [("p", C.c_int), ("q", C.c_int), ("rank", C.c_int), ("naxes", C.c_int),
+                ("z", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("limit", C.c_double), ("delta", C.c_double), ("p", C.c_double),
+                ("q", C.c_double), ("precision", C.c_double), ("log_scale", C.c_double),
+                ("valid", C.c_int32), ("n_limits", C.c_int32)]
+
+
+class Snaps(C.Structure):
+    _fields_ = [("m", C.c_double * (3 * 64)), ("ls", C.c_double * 3), ("dim", C.c_int),
+                ("valid", C.c_int)]
+
+
+RESULT_DTYPE = np.dtype([("limit", "f8"), ("delta", "f8"), ("p", "f8"), ("q", "f8"),
+                         ("precision", "f8"), ("log_scale", "f8"), ("valid", "i4"),
+                         ("n_limits", "i4")])
+SNAPS_DTYPE = np.dtype([("m", "f8", (3, 8, 8)), ("ls", "f8", (3,)), ("dim", "i4"),
+                        ("valid", "i4")])
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.orc_make_desc.restype = Desc
+        L.orc_make_desc.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.orc_llround.restype = C.c_int64
+        L.orc_llround.argtypes = [C.c_double]
+        L.orc_to_mod.restype = C.c_uint32
+        L.orc_to_mod.argtypes = [C.c_double, C.c_uint32]
+        L.orc_modulus.restype = C.c_uint32
+        L.orc_modulus.argtypes = [C.c_int]
+        L.orc_poly_theta_plus.argtypes = [P(C.c_double), C.c_int, P(C.c_double)]
+        L.orc_companion_column.restype = C.c_int
+        L.orc_companion_column.argtypes = [P(C.c_double), C.c_double, C.c_int, C.c_int, C.c_int,
+                                           P(C.c_double)]
+        L.orc_right_mul.restype = C.c_int
+        L.orc_right_mul.argtypes = [P(C.c_double), C.c_int, P(C.c_double), C.c_double, C.c_int,
+                                    P(C.c_double)]
+        L.orc_forward_walk.restype = C.c_int
+        L.orc_forward_walk.argtypes = [P(C.c_int32), P(C.c_double), P(Desc), C.c_int, C.c_int,
+                                       P(Snaps)]
+        L.orc_finish.argtypes = [P(Snaps), C.c_int, P(Result)]
+        L.orc_walk_f64_batch.argtypes = [P(Desc), C.c_int, P(C.c_int32), P(C.c_double), C.c_int,
+                                         C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_walk_rns.restype = C.c_int
+        L.orc_walk_rns.argtypes = [P(C.c_int32), P(C.c_double), P(Desc), C.c_int, C.c_int,
+                                   C.c_int, P(Result), P(Snaps), P(C.c_uint32)]
+        L.orc_walk_rns_batch.argtypes = [P(Desc), C.c_int, P(C.c_int32), P(C.c_double), C.c_int,
+                                         C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_rns_raw_walk.restype = C.c_int
+        L.orc_rns_raw_walk.argtypes = [P(C.c_int32), P(C.c_double), P(Desc), C.c_int, C.c_int,
+                                       P(C.c_uint32)]
+        L.orc_rns_right_mul_lane.restype = C.c_int
+        L.orc_rns_right_mul_lane.argtypes = [P(C.c_uint32), C.c_int, P(C.c_uint32), C.c_uint32,
+                                             C.c_int, C.c_uint32]
+        L.orc_mod_pow.restype = C.c_uint32
+        L.orc_mod_pow.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_crt_entry.restype = C.c_double
+        L.orc_crt_entry.argtypes = [C.c_uint32, C.c_uint32, C.c_int, P(C.c_uint64)]
+        L.orc_legacy_inputs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        P(C.c_int32), P(C.c_double)]
+        L.orc_mt64_seed.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_mt64_next.restype = C.c_uint64
+        L.orc_mt64_next.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref not built")
+        R = C.CDLL(REF_PATH)
+        P = C.POINTER
+        R.ref_walk_f64.restype = C.c_int
+        R.ref_walk_f64.argtypes = [C.c_int, C.c_int, C.c_double, P(C.c_int32), P(C.c_double),
+                                   C.c_int, C.c_int, P(C.c_double), P(C.c_int32), P(C.c_double)]
+        R.ref_walk_f64_batch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, P(C.c_int32),
+                                         P(C.c_double), C.c_int, C.c_int, P(C.c_double),
+                                         P(C.c_int32), P(C.c_int32), C.c_int]
+        _ref = R
+    return _ref
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def make_desc(p, q, z) -> Desc:
+    return lib().orc_make_desc(p, q, z)
+
+
+def pad16(x, dtype):
+    out = np.zeros(MAX_AXES, dtype=dtype)
+    out[:len(x)] = x
+    return out
+
+
+def as_traj_major(dirs, starts):
+    """dirs/starts as [n][16] contiguous arrays (int32 / float64)."""
+    dirs = np.asarray(dirs)
+    starts = np.asarray(starts, dtype=np.float64)
+    n = dirs.shape[0]
+    D = np.zeros((n, MAX_AXES), np.int32)
+    S = np.zeros((n, MAX_AXES), np.float64)
+    D[:, :dirs.shape[1]] = dirs
+    S[:, :starts.shape[1]] = starts
+    return np.ascontiguousarray(D), np.ascontiguousarray(S)
+
+
+def walk_f64_batch(p, q, z, dirs, starts, N, h, nthreads=0, want_snaps=False):
+    d = make_desc(p, q, z)
+    D, S = as_traj_major(dirs, starts)
+    n = D.shape[0]
+    res = np.zeros(n, RESULT_DTYPE)
+    snaps = np.zeros(n, SNAPS_DTYPE) if want_snaps else None
+    lib().orc_walk_f64_batch(C.byref(d), n, _iptr(D), _dptr(S), N, h,
+                             res.ctypes.data_as(C.c_void_p),
+                             snaps.ctypes.data_as(C.c_void_p) if want_snaps else None, nthreads)
+    return (res, snaps) if want_snaps else res
+
+
+def walk_rns_batch(p, q, z, dirs, starts, N, h, n_primes=8, nthreads=0, want_snaps=False):
+    d = make_desc(p, q, z)
+    D, S = as_traj_major(dirs, starts)
+    n = D.shape[0]
+    res = np.zeros(n, RESULT_DTYPE)
+    snaps = np.zeros(n, SNAPS_DTYPE) if want_snaps else None
+    lib().orc_walk_rns_batch(C.byref(d), n, _iptr(D), _dptr(S), N, h, n_primes,
+                             res.ctypes.data_as(C.c_void_p),
+                             snaps.ctypes.data_as(C.c_void_p) if want_snaps else None, nthreads)
+    return (res, snaps) if want_snaps else res
+
+
+def walk_rns_one(p, q, z, dir_, start, N, h, n_primes=8):
+    """One RNS trajectory: (result record, snaps record, final residues [np][8][8])."""
+    d = make_desc(p, q, z)
+    D = pad16(dir_, np.int32)
+    S = pad16(start, np.float64)
+    r = Result()
+    s = Snaps()
+    W = np.zeros((64, 8, 8), np.uint32)
+    lib().orc_walk_rns(_iptr(D), _dptr(S), C.byref(d), N, h, n_primes, C.byref(r), C.byref(s),
+                       _uptr(W))
+    return r, s, W[:n_primes]
+
+
+def rns_raw_walk(p, q, z, dir_, start, steps, n_primes=8):
+    d = make_desc(p, q, z)
+    D = pad16(dir_, np.int32)
+    S = pad16(start, np.float64)
+    W = np.zeros((64, 8, 8), np.uint32)
+    ok = lib().orc_rns_raw_walk(_iptr(D), _dptr(S), C.byref(d), steps, n_primes, _uptr(W))
+    return bool(ok), W[:n_primes]
+
+
+def ref_walk_f64_batch(p, q, z, dirs, starts, N, h, nthreads=0):
+    D, S = as_traj_major(dirs, starts)
+    n = D.shape[0]
+    out6 = np.zeros((n, 6))
+    nl = np.zeros(n, np.int32)
+    valid = np.zeros(n, np.int32)
+    ref().ref_walk_f64_batch(p, q, z, n, _iptr(D), _dptr(S), N, h, _dptr(out6), _iptr(nl),
+                             _iptr(valid), nthreads)
+    res = np.zeros(n, RESULT_DTYPE)
+    for k, name in enumerate(["limit", "delta", "p", "q", "precision", "log_scale"]):
+        res[name] = out6[:, k]
+    res["valid"] = valid
+    res["n_limits"] = nl
+    return res
+
+
+def ref_walk_f64_snaps(p, q, z, dir_, start, N, h):
+    D = pad16(dir_, np.int32)
+    S = pad16(start, np.float64)
+    out6 = np.zeros(6)
+    nl = np.zeros(1, np.int32)
+    sn = np.zeros(195)
+    ok = ref().ref_walk_f64(p, q, z, _iptr(D), _dptr(S), N, h, _dptr(out6), _iptr(nl), _dptr(sn))
+    return bool(ok), sn[:192].reshape(3, 8, 8), sn[192:195]
+
+
+def legacy_inputs(p, q, n, mode, use_zeta):
+    D = np.zeros((n, MAX_AXES), np.int32)
+    S = np.zeros((n, MAX_AXES), np.float64)
+    lib().orc_legacy_inputs(p, q, n, 1 if mode == "delta" else 0, 1 if use_zeta else 0, _iptr(D),
+                            _dptr(S))
+    return D, S
+
+
+class MT64:
+    """std::mt19937_64 through the oracle's C implementation."""
+
+    def __init__(self, seed):
+        self._buf = (C.c_uint64 * 313)()
+        lib().orc_mt64_seed(C.cast(self._buf, C.c_void_p), seed)
+
+    def __call__(self):
+        return int(lib().orc_mt64_next(C.cast(self._buf, C.c_void_p)))
+
+
+def modulus(k):
+    return int(lib().orc_modulus(k))
+
+
+# ------------------------------------------------------------ exact walk --
+
+def _poly_from_roots(vals):
+    """Integer/rational coefficients of prod (theta + v), ascending."""
+    c = [Fraction(1)]
+    for v in vals:
+        n = [Fraction(0)] * (len(c) + 1)
+        for k, ck in enumerate(c):
+            n[k] += ck * v
+            n[k + 1] += ck
+        c = n
+    return c
+
+
+def exact_axis_matrix(pos, z, dim, p, q):
+    """Dense theta-companion matrix C(pos) (Fractions) or None when the
+    leading coefficient of D(theta) = theta*prod(theta+y-1) - z*prod(theta+x)
+    vanishes.  Companion convention: ones on the subdiagonal, last column
+    -D[0..dim-1]/D[dim]."""
+    z = Fraction(z)
+    px = _poly_from_roots([Fraction(v) for v in pos[:p]])
+    py = _poly_from_roots([Fraction(v) - 1 for v in pos[p:p + q]])
+    D = [Fraction(0)] * (max(p, q + 1) + 2)
+    for k, c in enumerate(px):
+        D[k] += -z * c
+    for k, c in enumerate(py):
+        D[k + 1] += c
+    lead = D[dim]
+    if lead == 0:
+        return None
+    Cm = [[Fraction(0)] * dim for _ in range(dim)]
+    for i in range(1, dim):
+        Cm[i][i - 1] = Fraction(1)
+    for i in range(dim):
+        Cm[i][dim - 1] = -D[i] / lead
+    return Cm
+
+
+def _matmul(A, B):
+    n, m, k = len(A), len(B[0]), len(B)
+    return [[sum(A[i][t] * B[t][j] for t in range(k)) for j in range(m)] for i in range(n)]
+
+
+def _inverse(A):
+    n = len(A)
+    M = [row[:] + [Fraction(int(i == j)) for j in range(n)] for i, row in enumerate(A)]
+    for c in range(n):
+        piv = next((r for r in range(c, n) if M[r][c] != 0), None)
+        if piv is None:
+            return None
+        M[c], M[piv] = M[piv], M[c]
+        pv = M[c][c]
+        M[c] = [x / pv for x in M[c]]
+        for r in range(n):
+            if r != c and M[r][c] != 0:
+                f = M[r][c]
+                M[r] = [x - f * y for x, y in zip(M[r], M[c])]
+    return [row[n:] for row in M]
+
+
+def exact_walk(p, q, z, dir_, start, steps):
+    """Exact product W = prod over trajectory steps of the axis matrices, in
+    the reference's axis order, as Fractions.  Returns (W, ok)."""
+    d = make_desc(p, q, z)
+    dim, nax = d.rank, p + q
+    pos = [Fraction(v) for v in start[:nax]]
+    W = [[Fraction(int(i == j)) for j in range(dim)] for i in range(dim)]
+    I = [[Fraction(int(i == j)) for j in range(dim)] for i in range(dim)]
+    top = max(abs(int(v)) for v in dir_[:nax]) if nax else 0
+    for _ in range(steps):
+        for lev in range(top, 0, -1):
+            for a in range(nax - 1, -1, -1):
+                if abs(int(dir_[a])) < lev:
+                    continue
+                s = 1 if dir_[a] > 0 else -1
+                ev = list(pos)
+                inverse = (a < p and s < 0) or (a >= p and s > 0)
+                if a < p:
+                    a_den = pos[a] if s > 0 else pos[a] - 1
+                    if s < 0:
+                        ev[a] -= 1
+                else:
+                    a_den = pos[a] - 1 if s < 0 else pos[a]
+                    if s > 0:
+                        ev[a] += 1
+                Cm = exact_axis_matrix(ev, z, dim, p, q)
+                if Cm is None or a_den == 0:
+                    return W, False
+                A = [[I[i][j] + Cm[i][j] / a_den for j in range(dim)] for i in range(dim)]
+                if inverse:
+                    A = _inverse(A)
+                    if A is None:
+                        return W, False
+                W = _matmul(W, A)
+                pos[a] += s
+    return W, True
+
+
+def exact_log_abs(x: Fraction) -> float:
+    """log|x| for a (possibly huge) Fraction."""
+    n, d = abs(x.numerator), x.denominator
+    return (math.log(n) if n.bit_length() < 1000 else
+            (n.bit_length() - 60) * math.log(2) + math.log(n >> (n.bit_length() - 60))) - \
+        (math.log(d) if d.bit_length() < 1000 else
+         (d.bit_length() - 60) * math.log(2) + math.log(d >> (d.bit_length() - 60)))
