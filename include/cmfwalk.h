@@ -1,0 +1,148 @@
+/*
+ * cmfwalk.h -- C ABI of the MI355X pFq CMF walk engine (libcmfwalk.so).
+ *
+ * The reference has no library: each walk variant is a CUDA executable whose
+ * kernel is walk_kernel_{f64,rns,df64}(inputs*, results*, CMFDesc, N, h,
+ * n_traj[, n_primes]) and whose driver owns allocation, input generation,
+ * timing and JSON.  This header is the boundary those kernels sit behind,
+ * redrawn as plain C: caller-owned device buffers in structure-of-arrays
+ * form, asynchronous on a caller stream, an int status per call.  The
+ * cmf_walk_{f64,rns,df64} executables built on top keep the reference's
+ * command line and JSON (cuda/cmf_walk_f64.cu:318-357 and :263-305).
+ *
+ * Conventions
+ *  - All pointers in cmfw_inputs / cmfw_results are DEVICE pointers; arrays
+ *    are indexed [axis * ld + t] (inputs) or [t] (results), t < n_traj.
+ *  - Calls are stream-ordered on `stream` (a hipStream_t; NULL = default
+ *    stream) and return before the kernels finish.  They never synchronise
+ *    the device; temporary device memory comes from hipMallocAsync on the
+ *    same stream unless a workspace is passed.
+ *  - Return value: 0 on success, a negative CMFW_E* code otherwise; the
+ *    message is available from cmfw_last_error() (thread-local).
+ *  - Per-trajectory failures are not errors: they set valid = 0 and NaN
+ *    fields exactly as the reference does (cuda/cmf_walk_f64.cu:109-120).
+ */
+#ifndef CMFWALK_H
+#define CMFWALK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMFW_MAX_DIM 8
+#define CMFW_MAX_AXES 16
+#define CMFW_MAX_PRIMES 64
+
+enum {
+    CMFW_OK = 0,
+    CMFW_EINVAL = -1,   /* bad argument (sizes, pointers, p/q out of range) */
+    CMFW_EHIP = -2,     /* a HIP runtime call or kernel launch failed */
+    CMFW_ENOMEM = -3,   /* workspace too small / allocation failed */
+};
+
+/* CMF descriptor; replaces CMFDesc + make_cmf_desc (cuda/cmf_pfq.cuh:41-61).
+ * rank = max(p, q+1), minus 1 when z == 1 and p == q + 1. */
+typedef struct cmfw_desc {
+    int32_t p, q, rank, naxes;
+    double z;
+} cmfw_desc;
+
+/* Trajectory inputs; replaces TrajectoryInput {int dir[16]; Position start}
+ * (cuda/cmf_walk_f64.cu:36-39, RNSInput cuda/cmf_walk_rns.cu:85-88), stored
+ * as structure of arrays so one load instruction serves 64 trajectories. */
+typedef struct cmfw_inputs {
+    int32_t n_traj;
+    int32_t ld;            /* stride between axes, >= n_traj */
+    const int32_t *dir;    /* [naxes][ld] direction entries */
+    const double *start;   /* [naxes][ld] start point x_1..x_p, y_1..y_q */
+} cmfw_inputs;
+
+/* Per-trajectory results; replaces TrajectoryResult
+ * (cuda/cmf_walk_f64.cu:41-51) and RNSResult (cuda/cmf_walk_rns.cu:90-99;
+ * its p_f64/q_f64 are p/q here).  Any pointer may be NULL to skip it. */
+typedef struct cmfw_results {
+    double *limit;      /* first Aitken candidate, else the first candidate */
+    double *delta;      /* compute_kamidelta(snapshots, limit) */
+    double *p, *q;      /* M[0][rank-1], M[1][rank-1] at depth N, max-normalised */
+    double *precision;  /* -log10 |p/q(N) - p/q(N-h)| */
+    double *log_scale;  /* accumulated log of the normalisation scale at N */
+    int32_t *valid;
+    int32_t *n_limits;  /* candidate count of compute_limits, capped at 256 */
+    /* executed axis steps of each kind (I + C/a and its inverse), counting
+     * the failing one; they size the walk's arithmetic work.  Not in the
+     * reference. */
+    int32_t *n_fwd, *n_inv;
+} cmfw_results;
+
+/* Snapshot matrices at depths N-2h, N-h, N (cuda/cmf_pfq.cuh:336-344).
+ * m: [3][rank][rank][ld] doubles, ls: [3][ld].  Optional output. */
+typedef struct cmfw_snapshots {
+    int32_t ld;
+    double *m;
+    double *ls;
+} cmfw_snapshots;
+
+/* Library ------------------------------------------------------------- */
+
+const char *cmfw_last_error(void);
+const char *cmfw_version(void);
+/* desc from (p, q, z); CMFW_EINVAL when rank > 8 or p + q > 16 or p < 1. */
+int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out);
+/* Device bytes of temporary storage a walk of n_traj trajectories needs
+ * (snapshot staging); pass that much as `workspace` to avoid hipMallocAsync. */
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj);
+
+/* Plain f64 walk.  Replaces walk_kernel_f64 (cuda/cmf_walk_f64.cu:94-154):
+ * forward_walk + compute_limits + limit selection + compute_precision +
+ * compute_kamidelta (cuda/cmf_pfq.cuh:348-596), same operation order and
+ * IEEE rounding (no FMA contraction), same validity decisions.
+ * snaps may be NULL. */
+int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                  cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                  size_t workspace_bytes, void *stream);
+
+/* Prefix walk: the first K trajectory steps of the f64 walk above, with
+ * the same validity rules, for screening (a trajectory that fails within K
+ * steps fails the full walk identically).  Writes out->valid, out->n_fwd,
+ * out->n_inv and out->log_scale (log scale after K steps); other result
+ * pointers are ignored.  No counterpart in the reference, whose walk has no
+ * early exit. */
+int cmfw_walk_prefix_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K,
+                         cmfw_results *out, void *stream);
+
+/* RNS walk.  Replaces walk_kernel_rns (cuda/cmf_walk_rns.cu:260-320):
+ * n_primes residue lanes over the reference's modulus table, 2-prime Garner
+ * CRT every 100 steps and at N-2h, N-h, N, re-encode of the normalised
+ * doubles, then the same limit/delta reductions.  Residues and CRT values
+ * are bit-exact with the reference's integer arithmetic.
+ * resid_out (nullable, device): final residues [n_primes][rank][rank][n].
+ * crt_out (nullable, device): CRT values of the last snapshot
+ * [rank][rank][n] as uint64. */
+int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                  int32_t n_primes, cmfw_results *out, cmfw_snapshots *snaps,
+                  uint32_t *resid_out, uint64_t *crt_out, void *workspace,
+                  size_t workspace_bytes, void *stream);
+
+/* Synthetic inputs of the benchmark configurations, host memory, trajectory
+ * major [n][16] (unused axes zero): std::mt19937_64(seed), each value
+ * lo + rng() % (hi - lo + 1), trajectory by trajectory, axes x1..xp then
+ * y1..yq.  dir_lo/dir_hi draw directions; an all-zero direction gets
+ * dir[0] = 1 (the reference's rule, cuda/cmf_walk_f64.cu:84-89).  When
+ * start_lo > start_hi every start is the zeta point (1^p; 2^q)
+ * (cuda/cmf_pfq.cuh:73-79) and draws nothing.  Per trajectory the start
+ * values are drawn before the direction values. */
+int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t start_lo,
+                      int32_t start_hi, int32_t dir_lo, int32_t dir_hi, int32_t *dir_out,
+                      double *start_out);
+
+/* Modulus k of the reference table (cuda/cmf_walk_rns.cu:26-43), 0 if out
+ * of range. */
+uint32_t cmfw_modulus(int32_t k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -561,7 +561,7 @@ static void rns_identity(uint32_t *W, int np, int dim) {
 
 /* walk_kernel_rns, cmf_walk_rns.cu:260-319 */
 int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int N, int h,
-                 int np, orc_result *r, orc_snaps *s, uint32_t *resid_out) {
+                 int np, orc_result *r, orc_snaps *s, uint32_t *resid_out, uint64_t *crt_out) {
     uint32_t W[ORC_MAX_PRIMES * 64];
     const int dim = d->rank;
     rns_identity(W, np, dim);
@@ -580,10 +580,14 @@ int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int
         if (step % 100 == 0 || step == cp1 || step == cp2 || step == N) {
             double wm[64];
             memset(wm, 0, sizeof(wm));
+            if (step == N && resid_out) memcpy(resid_out, W, sizeof(uint32_t) * 64 * np);
             for (int i = 0; i < dim; ++i)
-                for (int j = 0; j < dim; ++j)
+                for (int j = 0; j < dim; ++j) {
+                    uint64_t v;
                     wm[i * 8 + j] = orc_crt_entry(W[i * 8 + j], np >= 2 ? W[64 + i * 8 + j] : 0,
-                                                  np, NULL);
+                                                  np, &v);
+                    if (step == N && crt_out) crt_out[i * 8 + j] = v;
+                }
             (void)normalize(wm, dim, &ls);
             if (step == cp1) { memcpy(s->m[0], wm, sizeof(wm)); s->ls[0] = ls; }
             if (step == cp2) { memcpy(s->m[1], wm, sizeof(wm)); s->ls[1] = ls; }
@@ -594,7 +598,6 @@ int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int
                         W[k * 64 + i * 8 + j] = orc_to_mod(wm[i * 8 + j], MODULI[k]);
         }
     }
-    if (resid_out) memcpy(resid_out, W, sizeof(uint32_t) * 64 * np);
     s->valid = ok;
     orc_finish(s, ok, r);
     return ok;
@@ -610,7 +613,7 @@ void orc_walk_rns_batch(const orc_desc *d, int n, const int32_t *dirs, const dou
     for (int i = 0; i < n; ++i) {
         orc_snaps s;
         orc_walk_rns(dirs + (size_t)i * 16, starts + (size_t)i * 16, d, N, h, np, &res[i], &s,
-                     NULL);
+                     NULL, NULL);
         if (snaps_out) snaps_out[i] = s;
     }
 }

@@ -77,13 +77,14 @@ void orc_finish(const orc_snaps *s, int ok, orc_result *r);
 void orc_walk_f64_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
                         int N, int h, orc_result *res, orc_snaps *snaps_out, int nthreads);
 
-/* RNS walk of one trajectory (cmf_walk_rns.cu:260-320).  If resid_out is
- * non-NULL it receives the final residues [n_primes][8][8] (before any
- * re-encode at step N is applied it is the re-encoded state, as the
- * reference leaves it).  crt_out, if non-NULL, receives the raw CRT value
- * (uint64) of every entry at the final snapshot [8][8]. */
+/* RNS walk of one trajectory (cmf_walk_rns.cu:260-320).  resid_out
+ * (nullable) receives the residues [n_primes][8][8] at step N just before
+ * the final CRT/re-encode; crt_out (nullable) the uint64 Garner value of
+ * every entry at step N [8][8].  Both are left untouched if the walk fails
+ * before step N. */
 int orc_walk_rns(const int32_t *dir, const double *start, const orc_desc *d, int N, int h,
-                 int n_primes, orc_result *r, orc_snaps *s, uint32_t *resid_out);
+                 int n_primes, orc_result *r, orc_snaps *s, uint32_t *resid_out,
+                 uint64_t *crt_out);
 void orc_walk_rns_batch(const orc_desc *d, int n, const int32_t *dirs, const double *starts,
                         int N, int h, int n_primes, orc_result *res, orc_snaps *snaps_out,
                         int nthreads);

@@ -85,7 +85,7 @@ def lib():
                                          C.c_int, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_walk_rns.restype = C.c_int
         L.orc_walk_rns.argtypes = [P(C.c_int32), P(C.c_double), P(Desc), C.c_int, C.c_int,
-                                   C.c_int, P(Result), P(Snaps), P(C.c_uint32)]
+                                   C.c_int, P(Result), P(Snaps), P(C.c_uint32), P(C.c_uint64)]
         L.orc_walk_rns_batch.argtypes = [P(Desc), C.c_int, P(C.c_int32), P(C.c_double), C.c_int,
                                          C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_rns_raw_walk.restype = C.c_int
@@ -187,16 +187,18 @@ def walk_rns_batch(p, q, z, dirs, starts, N, h, n_primes=8, nthreads=0, want_sna
 
 
 def walk_rns_one(p, q, z, dir_, start, N, h, n_primes=8):
-    """One RNS trajectory: (result record, snaps record, final residues [np][8][8])."""
+    """One RNS trajectory: (result, snaps, residues [np][8][8] at step N
+    before the final re-encode, CRT values [8][8] at step N)."""
     d = make_desc(p, q, z)
     D = pad16(dir_, np.int32)
     S = pad16(start, np.float64)
     r = Result()
     s = Snaps()
     W = np.zeros((64, 8, 8), np.uint32)
+    V = np.zeros((8, 8), np.uint64)
     lib().orc_walk_rns(_iptr(D), _dptr(S), C.byref(d), N, h, n_primes, C.byref(r), C.byref(s),
-                       _uptr(W))
-    return r, s, W[:n_primes]
+                       _uptr(W), V.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return r, s, W[:n_primes], V
 
 
 def rns_raw_walk(p, q, z, dir_, start, steps, n_primes=8):

@@ -1,0 +1,365 @@
+// cmf_walk.cpp -- the cmf_walk_{f64,rns,df64} command-line drop-ins.
+//
+// Same flags, defaults, exit codes, stderr lines and JSON keys as the
+// reference executables (argv: cuda/cmf_walk_f64.cu:318-357,
+// cuda/cmf_walk_rns.cu:414-434, cuda/cmf_walk_df64.cu:443-462; JSON:
+// cuda/cmf_walk_f64.cu:263-305, cuda/cmf_walk_rns.cu:396-407,
+// cuda/cmf_walk_df64.cu:425-436).  The walk itself runs in libcmfwalk.so.
+// Built three times with -DCMFW_VARIANT_{f64,rns,df64}.
+//
+// Additive flags (not in the reference):
+//   --inputs FILE    explicit trajectories (format in INPUT FILE below)
+//   --seed S         seed of the legacy generators (default 42)
+//   --clean-inputs   generate one record per trajectory instead of the
+//                    reference's stride-aliased layout
+//
+// INPUT FILE: 8-byte magic "CMFWIN1\0", int32 p, q, n, 0, then int32
+// dir[n][16] and double start[n][16] (trajectory-major, axes x1..xp,
+// y1..yq, unused axes zero).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cmfwalk.h"
+#include "legacy_inputs.hpp"
+
+#if defined(CMFW_VARIANT_f64)
+static const char *kVariant = "f64";
+static const int kDefaultBlock = 256;
+#elif defined(CMFW_VARIANT_rns)
+static const char *kVariant = "rns";
+static const int kDefaultBlock = 128;
+#elif defined(CMFW_VARIANT_df64)
+static const char *kVariant = "df64";
+static const int kDefaultBlock = 256;
+#else
+#error "define CMFW_VARIANT_f64, CMFW_VARIANT_rns or CMFW_VARIANT_df64"
+#endif
+
+static bool is_f64() { return strcmp(kVariant, "f64") == 0; }
+static bool is_rns() { return strcmp(kVariant, "rns") == 0; }
+
+#define HIP_OK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                           \
+            exit(2);                                                                     \
+        }                                                                                \
+    } while (0)
+
+// json_double (cuda/cmf_pfq.cuh:29-33): %.17g, or null when not finite
+static std::string jd(double x) {
+    if (!std::isfinite(x)) return "null";
+    char b[40];
+    snprintf(b, sizeof(b), "%.17g", x);
+    return b;
+}
+
+static bool read_inputs(const char *path, int p, int q, cmfw_cli::HostInputs &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    char magic[8];
+    int32_t hdr[4];
+    f.read(magic, 8);
+    f.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    if (!f || memcmp(magic, "CMFWIN1", 8) != 0) return false;
+    if (hdr[0] != p || hdr[1] != q || hdr[2] < 0) {
+        fprintf(stderr, "inputs file is %dF%d, run is %dF%d\n", hdr[0], hdr[1], p, q);
+        return false;
+    }
+    const int n = hdr[2], naxes = p + q;
+    std::vector<int32_t> d((size_t)n * 16);
+    std::vector<double> s((size_t)n * 16);
+    f.read(reinterpret_cast<char *>(d.data()), (std::streamsize)(d.size() * 4));
+    f.read(reinterpret_cast<char *>(s.data()), (std::streamsize)(s.size() * 8));
+    if (!f) return false;
+    out.n = n;
+    out.naxes = naxes;
+    out.dir.assign((size_t)naxes * n, 0);
+    out.start.assign((size_t)naxes * n, 0.0);
+    for (int t = 0; t < n; ++t)
+        for (int a = 0; a < naxes; ++a) {
+            out.dir[(size_t)a * n + t] = d[(size_t)t * 16 + a];
+            out.start[(size_t)a * n + t] = s[(size_t)t * 16 + a];
+        }
+    return true;
+}
+
+struct DeviceRun {
+    cmfw_inputs in{};
+    cmfw_results res{};
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    std::vector<void *> bufs;
+    template <typename T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, n * sizeof(T) + 16));
+        bufs.push_back(p);
+        return (T *)p;
+    }
+    ~DeviceRun() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+};
+
+static int launch(const cmfw_desc &d, const cmfw_inputs &in, int N, int h, int n_primes,
+                  cmfw_results &res, void *ws, size_t ws_bytes) {
+    if (is_rns())
+        return cmfw_walk_rns(&d, &in, N, h, n_primes, &res, nullptr, nullptr, nullptr, ws,
+                             ws_bytes, nullptr);
+#if defined(CMFW_VARIANT_df64)
+    return cmfw_walk_f32x2(&d, &in, N, h, &res, nullptr, ws, ws_bytes, nullptr);
+#else
+    return cmfw_walk_f64(&d, &in, N, h, &res, nullptr, ws, ws_bytes, nullptr);
+#endif
+}
+
+static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int mode,
+                         const char *output_path, int block_size, int n_primes, bool use_zeta,
+                         const char *inputs_path, uint64_t seed, bool clean) {
+    cmfw_desc desc;
+    if (cmfw_make_desc(p, q, z, &desc) != CMFW_OK) {
+        fprintf(stderr, "%s\n", cmfw_last_error());
+        return 1;
+    }
+    if (is_f64())
+        fprintf(stderr, "=== f64 Benchmark: %dF%d z=%.4f N=%d h=%d n_traj=%d rank=%d%s ===\n", p,
+                q, z, N, h, n_traj, desc.rank, use_zeta ? " [ZETA]" : "");
+    else if (is_rns())
+        fprintf(stderr, "=== RNS: %dF%d z=%.4f N=%d h=%d n_traj=%d rank=%d primes=%d%s ===\n", p,
+                q, z, N, h, n_traj, desc.rank, n_primes, use_zeta ? " [ZETA]" : "");
+    else
+        fprintf(stderr, "=== df64: %dF%d z=%.4f N=%d h=%d n_traj=%d rank=%d%s ===\n", p, q, z, N,
+                h, n_traj, desc.rank, use_zeta ? " [ZETA]" : "");
+
+    cmfw_cli::HostInputs hin;
+    if (inputs_path) {
+        if (!read_inputs(inputs_path, p, q, hin)) {
+            fprintf(stderr, "cannot read inputs file %s\n", inputs_path);
+            return 1;
+        }
+        n_traj = hin.n;
+    } else if (clean) {
+        cmfw_cli::make_clean_inputs(p, q, n_traj, mode, use_zeta, seed, hin);
+    } else {
+        cmfw_cli::make_legacy_inputs(p, q, n_traj, mode, use_zeta, seed, hin);
+    }
+    const int naxes = p + q;
+    const int n = n_traj;
+
+    DeviceRun R;
+    int32_t *d_dir = R.alloc<int32_t>((size_t)naxes * n);
+    double *d_start = R.alloc<double>((size_t)naxes * n);
+    HIP_OK(hipMemcpy(d_dir, hin.dir.data(), hin.dir.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_start, hin.start.data(), hin.start.size() * 8, hipMemcpyHostToDevice));
+    R.in.n_traj = n;
+    R.in.ld = n;
+    R.in.dir = d_dir;
+    R.in.start = d_start;
+    R.res.limit = R.alloc<double>(n);
+    R.res.delta = R.alloc<double>(n);
+    R.res.p = R.alloc<double>(n);
+    R.res.q = R.alloc<double>(n);
+    R.res.precision = R.alloc<double>(n);
+    R.res.log_scale = R.alloc<double>(n);
+    R.res.valid = R.alloc<int32_t>(n);
+    R.res.n_limits = R.alloc<int32_t>(n);
+    R.ws_bytes = cmfw_workspace_bytes(&desc, n);
+    R.ws = R.alloc<char>(R.ws_bytes);
+
+    // warm-up on one trajectory, as the reference does before timing
+    if (n > 0) {
+        cmfw_inputs one = R.in;
+        one.n_traj = 1;
+        if (launch(desc, one, N, h, n_primes, R.res, R.ws, R.ws_bytes) != CMFW_OK) {
+            fprintf(stderr, "walk failed: %s\n", cmfw_last_error());
+            return 2;
+        }
+        HIP_OK(hipDeviceSynchronize());
+    }
+    hipEvent_t ev0, ev1;
+    HIP_OK(hipEventCreate(&ev0));
+    HIP_OK(hipEventCreate(&ev1));
+    size_t mem_free = 0, mem_total = 0;
+    HIP_OK(hipEventRecord(ev0, nullptr));
+    if (launch(desc, R.in, N, h, n_primes, R.res, R.ws, R.ws_bytes) != CMFW_OK) {
+        fprintf(stderr, "walk failed: %s\n", cmfw_last_error());
+        return 2;
+    }
+    HIP_OK(hipEventRecord(ev1, nullptr));
+    HIP_OK(hipEventSynchronize(ev1));
+    HIP_OK(hipGetLastError());
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
+    HIP_OK(hipMemGetInfo(&mem_free, &mem_total));
+    const size_t mem_used = mem_total - mem_free;
+
+    std::vector<double> limit(n), delta(n), rp(n), rq(n), prec(n), lsc(n);
+    std::vector<int32_t> valid(n), nlim(n);
+    HIP_OK(hipMemcpy(limit.data(), R.res.limit, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(delta.data(), R.res.delta, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(rp.data(), R.res.p, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(rq.data(), R.res.q, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(prec.data(), R.res.precision, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(lsc.data(), R.res.log_scale, n * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(valid.data(), R.res.valid, n * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(nlim.data(), R.res.n_limits, n * 4, hipMemcpyDeviceToHost));
+
+    // statistics (cuda/cmf_walk_f64.cu:243-255)
+    int n_valid = 0, n_with_limits = 0;
+    double sum_delta = 0;
+    for (int i = 0; i < n; ++i)
+        if (valid[i]) {
+            n_valid++;
+            if (!std::isnan(limit[i])) n_with_limits++;
+            if (!std::isnan(delta[i])) sum_delta += delta[i];
+        }
+    const double avg_delta = (n_valid > 0) ? sum_delta / n_valid : NAN;
+    const double traj_per_s = n / (ms / 1000.0);
+
+    if (is_f64()) {
+        fprintf(stderr, "  Valid: %d/%d  With limits: %d  Avg delta: %.6f\n", n_valid, n,
+                n_with_limits, avg_delta);
+        fprintf(stderr, "  GPU time: %.3f ms  Throughput: %.0f traj/s  Mem used: %.2f MB\n", ms,
+                traj_per_s, mem_used / 1e6);
+    } else {
+        fprintf(stderr,
+                "  Valid: %d/%d  Limits: %d  Avg delta: %.6f  Time: %.3f ms  %.0f traj/s  Mem: "
+                "%.2f MB\n",
+                n_valid, n, n_with_limits, avg_delta, ms, traj_per_s, mem_used / 1e6);
+    }
+
+    std::ofstream out(output_path);
+    if (is_f64()) {
+        out << "{\n";
+        out << "  \"variant\": \"f64\",\n";
+        out << "  \"cmf\": \"" << p << "F" << q << "\",\n";
+        out << "  \"p\": " << p << ", \"q\": " << q << ",\n";
+        out << "  \"z\": " << z << ",\n";
+        out << "  \"rank\": " << desc.rank << ",\n";
+        out << "  \"naxes\": " << desc.naxes << ",\n";
+        out << "  \"N\": " << N << ",\n";
+        out << "  \"h\": " << h << ",\n";
+        out << "  \"n_traj\": " << n << ",\n";
+        out << "  \"mode\": \"" << (mode == 0 ? "limit" : "delta") << "\",\n";
+        out << "  \"block_size\": " << block_size << ",\n";
+        out << "  \"gpu_time_ms\": " << ms << ",\n";
+        out << "  \"throughput_traj_per_s\": " << traj_per_s << ",\n";
+        out << "  \"mem_used_mb\": " << mem_used / 1e6 << ",\n";
+        out << "  \"n_valid\": " << n_valid << ",\n";
+        out << "  \"n_with_limits\": " << n_with_limits << ",\n";
+        out << "  \"avg_delta\": "
+            << (std::isfinite(avg_delta) ? std::to_string(avg_delta) : "null") << ",\n";
+        out << "  \"results\": [\n";
+        const int nr = n < 100 ? n : 100;
+        for (int i = 0; i < nr; ++i) {
+            out << "    {\"idx\": " << i << ", \"valid\": " << valid[i]
+                << ", \"limit\": " << jd(limit[i]) << ", \"delta\": " << jd(delta[i])
+                << ", \"p\": " << jd(rp[i]) << ", \"q\": " << jd(rq[i])
+                << ", \"precision\": " << jd(prec[i]) << ", \"log_scale\": " << jd(lsc[i])
+                << ", \"n_limits\": " << nlim[i] << "}";
+            if (i < nr - 1) out << ",";
+            out << "\n";
+        }
+        out << "  ]\n";
+        out << "}\n";
+    } else {
+        out << "{\n  \"variant\": \"" << kVariant << "\",\n  \"cmf\": \"" << p << "F" << q
+            << "\",\n";
+        out << "  \"p\": " << p << ", \"q\": " << q << ", \"z\": " << z << ",\n";
+        out << "  \"rank\": " << desc.rank << ", \"naxes\": " << desc.naxes << ",\n";
+        out << "  \"N\": " << N << ", \"h\": " << h << ", \"n_traj\": " << n << ",\n";
+        out << "  \"mode\": \"" << (mode == 0 ? "limit" : "delta") << "\",\n";
+        if (is_rns())
+            out << "  \"block_size\": " << block_size << ", \"n_primes\": " << n_primes << ",\n";
+        else
+            out << "  \"block_size\": " << block_size << ",\n";
+        out << "  \"gpu_time_ms\": " << ms << ", \"throughput_traj_per_s\": " << traj_per_s
+            << ",\n";
+        out << "  \"mem_used_mb\": " << mem_used / 1e6 << ",\n";
+        out << "  \"n_valid\": " << n_valid << ", \"n_with_limits\": " << n_with_limits
+            << ", \"avg_delta\": " << jd(avg_delta) << "\n}\n";
+    }
+    out.close();
+    if (is_f64()) fprintf(stderr, "  Results written to: %s\n", output_path);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    int p = 6, q = 5;
+    double z = 0.5;
+    int N = 1000, h = 50, n_traj = 10000;
+    int mode = 0, block_size = kDefaultBlock, n_primes = 8;
+    bool use_zeta = false, clean = false;
+    uint64_t seed = 42;
+    const char *inputs_path = nullptr;
+    std::string def_out = std::string("results_") + kVariant + ".json";
+    const char *output_path = def_out.c_str();
+
+    for (int i = 1; i < argc; ++i) {
+        std::string arg = argv[i];
+        auto next = [&]() -> const char * {
+            if (++i >= argc) {
+                if (is_f64()) fprintf(stderr, "Missing value after %s\n", argv[i - 1]);
+                exit(1);
+            }
+            return argv[i];
+        };
+        if (arg == "--cmf" && i + 1 < argc) {
+            if (sscanf(argv[++i], "%dF%d", &p, &q) != 2) {
+                if (is_f64()) fprintf(stderr, "Invalid CMF type: %s (use e.g. 6F5)\n", argv[i]);
+                return 1;
+            }
+        } else if (arg == "--z") z = atof(next());
+        else if (arg == "--N") N = atoi(next());
+        else if (arg == "--h") h = atoi(next());
+        else if (arg == "--n-traj") n_traj = atoi(next());
+        else if (arg == "--mode") mode = (strcmp(next(), "delta") == 0) ? 1 : 0;
+        else if (arg == "--block-size") block_size = atoi(next());
+        else if (arg == "--output") output_path = next();
+        else if (arg == "--zeta") {
+            use_zeta = true;
+            z = 1.0;
+        } else if (is_rns() && arg == "--n-primes") n_primes = atoi(next());
+        else if (arg == "--inputs") inputs_path = next();
+        else if (arg == "--seed") seed = strtoull(next(), nullptr, 10);
+        else if (arg == "--clean-inputs") clean = true;
+        else {
+            fprintf(stderr, is_f64() ? "Unknown arg: %s\n" : "Unknown: %s\n", argv[i]);
+            return 1;
+        }
+    }
+    if (n_traj < 0) {
+        fprintf(stderr, "n_traj must be >= 0\n");
+        return 1;
+    }
+
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    if (is_f64()) {
+        fprintf(stderr, "GPU: %s (SM %d.%d, %d MPs, %d threads/MP, %.0f MHz)\n", prop.name,
+                prop.major, prop.minor, prop.multiProcessorCount, prop.maxThreadsPerMultiProcessor,
+                prop.clockRate / 1e3);
+        fprintf(stderr, "FP64: %s, %.1f GB global mem\n",
+                prop.computeMode == hipComputeModeDefault ? "supported" : "exclusive",
+                prop.totalGlobalMem / 1e9);
+    } else {
+        fprintf(stderr, "GPU: %s (SM %d.%d, %d MPs)\n", prop.name, prop.major, prop.minor,
+                prop.multiProcessorCount);
+    }
+    return run_benchmark(p, q, z, N, h, n_traj, mode, output_path, block_size, n_primes, use_zeta,
+                         inputs_path, seed, clean);
+}

@@ -1,0 +1,255 @@
+"""Python binding of libcmfwalk.so, the MI355X pFq CMF walk engine.
+
+Thin ctypes layer over the C ABI in include/cmfwalk.h.  PyTorch is used only
+for device memory and the current HIP stream; all walking happens in the
+library's hand-written gfx950 kernels.  There is no fallback: when the
+library is missing, or no GPU is visible, ``load()`` raises.
+
+Inputs follow the engine's structure-of-arrays layout: ``dir`` int32 and
+``start`` float64 tensors of shape [naxes, n] on the GPU (numpy arrays of
+shape [n, naxes] are accepted and uploaded).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcmfwalk.so")
+
+_lib = None
+
+
+class Desc(C.Structure):
+    _fields_ = [("p", C.c_int32), ("q", C.c_int32), ("rank", C.c_int32), ("naxes", C.c_int32),
+                ("z", C.c_double)]
+
+
+class Inputs(C.Structure):
+    _fields_ = [("n_traj", C.c_int32), ("ld", C.c_int32), ("dir", C.c_void_p),
+                ("start", C.c_void_p)]
+
+
+class Results(C.Structure):
+    _fields_ = [("limit", C.c_void_p), ("delta", C.c_void_p), ("p", C.c_void_p),
+                ("q", C.c_void_p), ("precision", C.c_void_p), ("log_scale", C.c_void_p),
+                ("valid", C.c_void_p), ("n_limits", C.c_void_p), ("n_fwd", C.c_void_p),
+                ("n_inv", C.c_void_p)]
+
+
+class Snapshots(C.Structure):
+    _fields_ = [("ld", C.c_int32), ("m", C.c_void_p), ("ls", C.c_void_p)]
+
+
+RESULT_FIELDS = ["limit", "delta", "p", "q", "precision", "log_scale", "valid", "n_limits",
+                 "n_fwd", "n_inv"]
+INT_FIELDS = ("valid", "n_limits", "n_fwd", "n_inv")
+# C entry points and the argument lists a ctypes caller binds
+SIGNATURES = {
+    "cmfw_last_error": (C.c_char_p, []),
+    "cmfw_version": (C.c_char_p, []),
+    "cmfw_make_desc": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.POINTER(Desc)]),
+    "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32]),
+    "cmfw_modulus": (C.c_uint32, [C.c_int32]),
+    "cmfw_walk_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
+                                C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p, C.c_size_t,
+                                C.c_void_p]),
+    "cmfw_walk_prefix_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32,
+                                       C.POINTER(Results), C.c_void_p]),
+    "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "cmfw_walk_rns": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
+                                C.c_int32, C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+}
+
+
+class CmfwError(RuntimeError):
+    pass
+
+
+def load_library():
+    """Load libcmfwalk.so and bind every entry point (no GPU needed)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CmfwError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def load():
+    """Load the library and require a visible GPU; raises otherwise."""
+    import torch
+
+    L = load_library()
+    if not torch.cuda.is_available():
+        raise CmfwError("no GPU visible: the CMF walk engine runs only on the MI355X")
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise CmfwError(f"cmfwalk error {rc}: {_lib.cmfw_last_error().decode()}")
+
+
+def make_desc(p, q, z) -> Desc:
+    d = Desc()
+    _check(load_library().cmfw_make_desc(p, q, float(z), C.byref(d)))
+    return d
+
+
+def modulus(k) -> int:
+    return int(load_library().cmfw_modulus(k))
+
+
+def _stream_ptr(stream):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def to_device_inputs(dirs, starts, device="cuda"):
+    """[n, naxes] host arrays -> ([naxes, n] int32, [naxes, n] float64) GPU tensors."""
+    import torch
+
+    if isinstance(dirs, torch.Tensor) and dirs.is_cuda:
+        return dirs, starts
+    d = np.ascontiguousarray(np.asarray(dirs, dtype=np.int32).T)
+    s = np.ascontiguousarray(np.asarray(starts, dtype=np.float64).T)
+    return (torch.from_numpy(d).to(device), torch.from_numpy(s).to(device))
+
+
+def alloc_results(n, device="cuda"):
+    import torch
+
+    out = {}
+    for f in RESULT_FIELDS:
+        dt = torch.int32 if f in INT_FIELDS else torch.float64
+        out[f] = torch.empty(n, dtype=dt, device=device)
+    return out
+
+
+def _results_struct(out):
+    r = Results()
+    for f in RESULT_FIELDS:
+        setattr(r, f, out[f].data_ptr() if out.get(f) is not None else None)
+    return r
+
+
+def _inputs_struct(dir_t, start_t):
+    n = dir_t.shape[1]
+    assert dir_t.dtype.is_floating_point is False and start_t.dtype.is_floating_point
+    assert dir_t.is_contiguous() and start_t.is_contiguous()
+    i = Inputs()
+    i.n_traj = n
+    i.ld = n
+    i.dir = dir_t.data_ptr()
+    i.start = start_t.data_ptr()
+    return i
+
+
+def _snap_struct(snap):
+    if snap is None:
+        return None
+    s = Snapshots()
+    s.ld = snap["m"].shape[-1]
+    s.m = snap["m"].data_ptr()
+    s.ls = snap["ls"].data_ptr()
+    return s
+
+
+def alloc_snapshots(rank, n, device="cuda"):
+    import torch
+
+    return {"m": torch.empty((3, rank, rank, n), dtype=torch.float64, device=device),
+            "ls": torch.empty((3, n), dtype=torch.float64, device=device)}
+
+
+def walk_f64(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspace=None,
+             stream=None):
+    """Plain f64 walk of every trajectory; returns the result tensors."""
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    out = out if out is not None else alloc_results(n, dt.device)
+    ws, wsb = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel())
+    sn = _snap_struct(snapshots)
+    _check(L.cmfw_walk_f64(C.byref(d), C.byref(_inputs_struct(dt, st)), N, h,
+                           C.byref(_results_struct(out)), C.byref(sn) if sn else None, ws, wsb,
+                           _stream_ptr(stream)))
+    return out
+
+
+def walk_prefix_f64(p, q, z, dirs, starts, K, out=None, stream=None):
+    """First K trajectory steps of the f64 walk: valid, n_fwd, n_inv and the
+    log scale after K steps."""
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    out = out if out is not None else alloc_results(n, dt.device)
+    _check(L.cmfw_walk_prefix_f64(C.byref(d), C.byref(_inputs_struct(dt, st)), K,
+                                  C.byref(_results_struct(out)), _stream_ptr(stream)))
+    return out
+
+
+def synth_inputs(seed, n, p, q, start_lo, start_hi, dir_lo, dir_hi):
+    """Host arrays ([n, 16] int32 dirs, [n, 16] float64 starts) from the
+    library's mt19937_64 generator (start_lo > start_hi: zeta start)."""
+    L = load_library()
+    D = np.zeros((n, 16), np.int32)
+    S = np.zeros((n, 16), np.float64)
+    _check(L.cmfw_synth_inputs(seed, n, p, q, start_lo, start_hi, dir_lo, dir_hi,
+                               D.ctypes.data, S.ctypes.data))
+    return D, S
+
+
+def walk_rns(p, q, z, dirs, starts, N, h, n_primes=8, out=None, snapshots=None,
+             want_resid=False, want_crt=False, workspace=None, stream=None):
+    """RNS walk (residues bit-exact with the reference's integer arithmetic).
+    With want_resid/want_crt the final residues [np][rank][rank][n] (uint32
+    stored as int32) and CRT values [rank][rank][n] (uint64 stored as int64)
+    at step N are returned as well."""
+    import torch
+
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    out = out if out is not None else alloc_results(n, dt.device)
+    resid = torch.zeros((n_primes, d.rank, d.rank, n), dtype=torch.int32, device=dt.device) \
+        if want_resid else None
+    crt = torch.zeros((d.rank, d.rank, n), dtype=torch.int64, device=dt.device) \
+        if want_crt else None
+    ws, wsb = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel())
+    sn = _snap_struct(snapshots)
+    _check(L.cmfw_walk_rns(C.byref(d), C.byref(_inputs_struct(dt, st)), N, h, n_primes,
+                           C.byref(_results_struct(out)), C.byref(sn) if sn else None,
+                           resid.data_ptr() if resid is not None else None,
+                           crt.data_ptr() if crt is not None else None, ws, wsb,
+                           _stream_ptr(stream)))
+    if want_resid or want_crt:
+        return out, resid, crt
+    return out
+
+
+def results_to_numpy(out):
+    """Result tensors -> numpy structured array with the oracle's field names."""
+    import torch
+
+    torch.cuda.synchronize()
+    n = out["valid"].numel()
+    arr = np.zeros(n, dtype=[(f, "i4" if f in INT_FIELDS else "f8") for f in RESULT_FIELDS])
+    for f in RESULT_FIELDS:
+        arr[f] = out[f].cpu().numpy()
+    return arr

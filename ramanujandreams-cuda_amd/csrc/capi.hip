@@ -1,0 +1,265 @@
+// capi.hip -- the extern "C" boundary of libcmfwalk.so (include/cmfwalk.h).
+//
+// Validates arguments on the host, lays out temporary snapshot storage in
+// the caller's workspace (or in stream-ordered memory from hipMallocAsync),
+// and launches the kernels on the caller's stream.  Nothing here blocks the
+// host on the device.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <random>
+
+#include "../../include/cmfwalk.h"
+#include "walk_common.hpp"
+#include "walk_kernels.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(CMFW_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_desc(const cmfw_desc *d) {
+    if (!d) return fail(CMFW_EINVAL, "desc is NULL");
+    if (d->p < 1 || d->q < 0 || d->p + d->q > CMFW_MAX_AXES || d->naxes != d->p + d->q)
+        return fail(CMFW_EINVAL, "unsupported CMF %dF%d (need p >= 1, p + q <= 16)", d->p, d->q);
+    if (d->rank < 2 || d->rank > CMFW_MAX_DIM)
+        return fail(CMFW_EINVAL, "rank %d of %dF%d z=%g outside 2..8", d->rank, d->p, d->q, d->z);
+    if (d->p > 8 || d->q > 8) return fail(CMFW_EINVAL, "p and q must be <= 8");
+    return CMFW_OK;
+}
+
+int check_inputs(const cmfw_inputs *in) {
+    if (!in) return fail(CMFW_EINVAL, "inputs is NULL");
+    if (in->n_traj < 0) return fail(CMFW_EINVAL, "n_traj < 0");
+    if (in->n_traj > 0 && (!in->dir || !in->start))
+        return fail(CMFW_EINVAL, "inputs dir/start is NULL");
+    if (in->ld < in->n_traj) return fail(CMFW_EINVAL, "ld %d < n_traj %d", in->ld, in->n_traj);
+    return CMFW_OK;
+}
+
+size_t snap_bytes(int D, int n) {
+    // [3][D][D][n] matrices + [3][n] log scales, 256-byte aligned pieces
+    size_t m = (size_t)3 * D * D * (size_t)n * sizeof(double);
+    size_t l = (size_t)3 * (size_t)n * sizeof(double);
+    return ((m + 255) & ~(size_t)255) + ((l + 255) & ~(size_t)255);
+}
+
+// Scratch snapshot storage: the caller's snaps, the caller's workspace, or
+// stream-ordered device memory released on the same stream afterwards.
+struct SnapStore {
+    double *m = nullptr, *ls = nullptr;
+    int ld = 0;
+    void *owned = nullptr;
+};
+
+int get_snaps(int D, int n, cmfw_snapshots *snaps, void *ws, size_t ws_bytes, hipStream_t st,
+              SnapStore &S) {
+    if (snaps && snaps->m && snaps->ls) {
+        if (snaps->ld < n) return fail(CMFW_EINVAL, "snapshots ld %d < n_traj %d", snaps->ld, n);
+        S.m = snaps->m;
+        S.ls = snaps->ls;
+        S.ld = snaps->ld;
+        return CMFW_OK;
+    }
+    const size_t need = snap_bytes(D, n);
+    void *base = nullptr;
+    if (ws) {
+        if (ws_bytes < need)
+            return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", ws_bytes, need);
+        base = ws;
+    } else {
+        hipError_t e = hipMallocAsync(&base, need, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(snapshots)");
+        S.owned = base;
+    }
+    size_t m = (size_t)3 * D * D * (size_t)n * sizeof(double);
+    S.m = (double *)base;
+    S.ls = (double *)((char *)base + ((m + 255) & ~(size_t)255));
+    S.ld = n;
+    return CMFW_OK;
+}
+
+int release(SnapStore &S, hipStream_t st, int rc) {
+    if (S.owned) {
+        hipError_t e = hipFreeAsync(S.owned, st);
+        S.owned = nullptr;
+        if (rc == CMFW_OK && e != hipSuccess) return hip_fail(e, "hipFreeAsync");
+    }
+    return rc;
+}
+
+void fill_args(cmfw::WalkArgs &A, const cmfw_desc *d, const cmfw_inputs *in, int N, int h,
+               cmfw_results *out, const SnapStore &S) {
+    memset(&A, 0, sizeof(A));
+    A.p = d->p;
+    A.q = d->q;
+    A.naxes = d->naxes;
+    A.z = d->z;
+    A.n = in->n_traj;
+    A.ld = in->ld;
+    A.N = N;
+    A.h = h;
+    A.dir = in->dir;
+    A.start = in->start;
+    A.snap = S.m;
+    A.snap_ls = S.ls;
+    A.sld = S.ld;
+    if (out) {
+        A.limit = out->limit;
+        A.delta = out->delta;
+        A.rp = out->p;
+        A.rq = out->q;
+        A.precision = out->precision;
+        A.log_scale = out->log_scale;
+        A.valid = out->valid;
+        A.n_limits = out->n_limits;
+        A.n_fwd = out->n_fwd;
+        A.n_inv = out->n_inv;
+    }
+}
+
+const uint32_t kHostModuli[CMFW_MAX_PRIMES] = {
+    1073741789u, 1073741783u, 1073741741u, 1073741723u, 1073741717u, 1073741707u,
+    1073741693u, 1073741683u, 1073741669u, 1073741657u, 1073741641u, 1073741623u,
+    1073741611u, 1073741597u, 1073741587u, 1073741573u, 1073741563u, 1073741551u,
+    1073741537u, 1073741527u, 1073741513u, 1073741503u, 1073741491u, 1073741477u,
+    1073741467u, 1073741453u, 1073741441u, 1073741427u, 1073741413u, 1073741403u,
+    1073741391u, 1073741377u, 1073741363u, 1073741353u, 1073741341u, 1073741329u,
+    1073741317u, 1073741307u, 1073741293u, 1073741281u, 1073741267u, 1073741257u,
+    1073741243u, 1073741231u, 1073741219u, 1073741207u, 1073741193u, 1073741183u,
+    1073741171u, 1073741157u, 1073741147u, 1073741137u, 1073741123u, 1073741111u,
+    1073741097u, 1073741087u, 1073741077u, 1073741063u, 1073741051u, 1073741039u,
+    1073741027u, 1073741017u, 1073741003u, 1073740991u};
+
+}  // namespace
+
+extern "C" {
+
+const char *cmfw_last_error(void) { return g_err; }
+
+const char *cmfw_version(void) { return "cmfwalk 0.1 gfx950"; }
+
+int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out) {
+    if (!out) return fail(CMFW_EINVAL, "out is NULL");
+    cmfw_desc d;
+    d.p = p;
+    d.q = q;
+    d.z = z;
+    d.naxes = p + q;
+    d.rank = (p > q + 1) ? p : q + 1;
+    if (z == 1.0 && p == q + 1) d.rank -= 1;
+    *out = d;
+    return check_desc(&d);
+}
+
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj) {
+    if (!desc || n_traj < 0) return 0;
+    return snap_bytes(desc->rank, n_traj);
+}
+
+uint32_t cmfw_modulus(int32_t k) {
+    return (k >= 0 && k < CMFW_MAX_PRIMES) ? kHostModuli[k] : 0u;
+}
+
+int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                  cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                  size_t workspace_bytes, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (in->n_traj == 0) return CMFW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    SnapStore S;
+    if ((rc = get_snaps(desc->rank, in->n_traj, snaps, workspace, workspace_bytes, st, S)))
+        return rc;
+    cmfw::WalkArgs A;
+    fill_args(A, desc, in, N, h, out, S);
+    hipError_t e = cmfw::launch_walk_f64_ref(desc->rank, A, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_f64 launch");
+    return release(S, st, rc);
+}
+
+int cmfw_walk_prefix_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K,
+                         cmfw_results *out, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (K < 1) return fail(CMFW_EINVAL, "prefix length %d < 1", K);
+    if (in->n_traj == 0) return CMFW_OK;
+    cmfw::WalkArgs A;
+    SnapStore S;  // no snapshots in prefix mode
+    cmfw_results r{};
+    if (out) {
+        r.valid = out->valid;
+        r.n_fwd = out->n_fwd;
+        r.n_inv = out->n_inv;
+        r.log_scale = out->log_scale;
+    }
+    fill_args(A, desc, in, K, 0, &r, S);
+    A.prefix = K;
+    hipError_t e = cmfw::launch_walk_f64_ref(desc->rank, A, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_prefix_f64 launch");
+}
+
+int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t start_lo,
+                      int32_t start_hi, int32_t dir_lo, int32_t dir_hi, int32_t *dir_out,
+                      double *start_out) {
+    if (n < 0 || p < 1 || q < 0 || p + q > CMFW_MAX_AXES || !dir_out || !start_out ||
+        dir_lo > dir_hi)
+        return fail(CMFW_EINVAL, "bad synth_inputs arguments");
+    std::mt19937_64 rng(seed);
+    const int naxes = p + q;
+    const bool zeta = start_lo > start_hi;
+    const uint64_t sspan = zeta ? 1 : (uint64_t)(start_hi - start_lo + 1);
+    const uint64_t dspan = (uint64_t)(dir_hi - dir_lo + 1);
+    for (int64_t t = 0; t < n; ++t) {
+        int32_t *d = dir_out + t * 16;
+        double *s = start_out + t * 16;
+        for (int a = 0; a < 16; ++a) {
+            d[a] = 0;
+            s[a] = 0.0;
+        }
+        for (int a = 0; a < naxes; ++a)
+            s[a] = zeta ? (a < p ? 1.0 : 2.0) : (double)(start_lo + (int64_t)(rng() % sspan));
+        bool zero = true;
+        for (int a = 0; a < naxes; ++a) {
+            d[a] = dir_lo + (int32_t)(rng() % dspan);
+            zero = zero && d[a] == 0;
+        }
+        if (zero) d[0] = 1;
+    }
+    return CMFW_OK;
+}
+
+int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                  int32_t n_primes, cmfw_results *out, cmfw_snapshots *snaps,
+                  uint32_t *resid_out, uint64_t *crt_out, void *workspace,
+                  size_t workspace_bytes, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (n_primes < 1 || n_primes > CMFW_MAX_PRIMES)
+        return fail(CMFW_EINVAL, "n_primes %d outside 1..64", n_primes);
+    if (in->n_traj == 0) return CMFW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    SnapStore S;
+    if ((rc = get_snaps(desc->rank, in->n_traj, snaps, workspace, workspace_bytes, st, S)))
+        return rc;
+    cmfw::WalkArgs A;
+    fill_args(A, desc, in, N, h, out, S);
+    hipError_t e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_rns launch");
+    return release(S, st, rc);
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// walk_common.hpp -- device pieces shared by the CMF walk kernels (gfx950).
+//
+// Trajectory-per-lane data layout in HBM (structure of arrays, so a wave's
+// 64 lanes touch 64 consecutive elements per load/store):
+//   dir   int32  [naxes][ld]           start double [naxes][ld]
+//   snap  double [3][D][D][sld]        snap_ls double [3][sld]
+//   results: one double / int32 array per field, [n]
+// Per-lane state lives in VGPRs: the D x D walk matrix, the x/y positions
+// split into two 8-entry arrays (so every register index is a compile-time
+// constant), and the 16 direction entries.  Axis indices are uniform across
+// the wave and are resolved with unrolled compare/select chains, never with
+// indexed register access (which would spill to scratch).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cmfw {
+
+constexpr double kEps = 2.220446049250313e-16;
+
+struct WalkArgs {
+    int p, q, naxes, n, ld;
+    double z;
+    int N, h;
+    const int32_t *dir;
+    const double *start;
+    double *snap;     // [3][D][D][sld]
+    double *snap_ls;  // [3][sld]
+    int sld;
+    // results (nullable)
+    double *limit, *delta, *rp, *rq, *precision, *log_scale;
+    int32_t *valid, *n_limits, *n_fwd, *n_inv;
+    int prefix;  // > 0: walk only this many trajectory steps (screening)
+};
+
+template <typename T, int K>
+__device__ __forceinline__ T pick(const T (&a)[K], int idx) {
+    T r = a[0];
+#pragma unroll
+    for (int j = 1; j < K; ++j) r = (j == idx) ? a[j] : r;
+    return r;
+}
+
+__device__ __forceinline__ double snap_at(const WalkArgs &A, int D, int s, int i, int j, int t) {
+    return A.snap[((size_t)((s * D + i) * D + j)) * A.sld + t];
+}
+
+// --------------------------------------------------------------------------
+// Snapshot reductions, one lane per trajectory.  Operation order follows
+// compute_limits / compute_delta / compute_precision / compute_kamidelta
+// (cuda/cmf_pfq.cuh:385-596) and the limit selection of walk_kernel_f64
+// (cuda/cmf_walk_f64.cu:128-153).  Only the candidate count (capped at 256)
+// and the first Aitken / first candidate are kept: that is all the drivers
+// read from the candidate list.
+// --------------------------------------------------------------------------
+struct LimitTally {
+    int n;
+    double first, first_aitken;
+    bool has_aitken;
+    __device__ void push(double L, bool aitken) {
+        if (n < 256) {
+            if (n == 0) first = L;
+            if (aitken && !has_aitken) {
+                first_aitken = L;
+                has_aitken = true;
+            }
+            ++n;
+        }
+    }
+};
+
+template <int D>
+__device__ void finish_from_snapshots(const WalkArgs &A, int t, bool ok) {
+#pragma clang fp contract(off)
+    if (!ok) {
+        if (A.valid) A.valid[t] = 0;
+        const double nan = __builtin_nan("");
+        if (A.limit) A.limit[t] = nan;
+        if (A.delta) A.delta[t] = nan;
+        if (A.rp) A.rp[t] = nan;
+        if (A.rq) A.rq[t] = nan;
+        if (A.precision) A.precision[t] = nan;
+        if (A.log_scale) A.log_scale[t] = nan;
+        if (A.n_limits) A.n_limits[t] = 0;
+        return;
+    }
+    const double DEN_MIN = 1e-280;
+    LimitTally T{0, 0.0, 0.0, false};
+    const int lc = D - 1;
+    // primary projection p = M[0][D-1], q = M[1][D-1]
+    const double p3 = snap_at(A, D, 2, 0, lc, t), q3 = snap_at(A, D, 2, 1, lc, t);
+    const double p2 = snap_at(A, D, 1, 0, lc, t), q2 = snap_at(A, D, 1, 1, lc, t);
+    const double p1 = snap_at(A, D, 0, 0, lc, t), q1 = snap_at(A, D, 0, 1, lc, t);
+    if (fabs(q3) > DEN_MIN && fabs(q2) > DEN_MIN && fabs(q1) > DEN_MIN) {
+        double r1 = p1 / q1, r2 = p2 / q2, r3 = p3 / q3;
+        if (isfinite(r1) && isfinite(r2) && isfinite(r3)) {
+            double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+            double den = r1 - 2.0 * r2 + r3;
+            if (fabs(den) > 1e-14 * sc) {
+                double dd = r2 - r3;
+                double L = r3 - dd * dd / den;
+                if (isfinite(L)) T.push(L, true);
+            }
+            if (fabs(r3) > 1e-15) T.push(r3, false);
+        }
+    }
+    // secondary scan over (column, i, j), i != j, skipping the primary
+    for (int c = 0; c < D; ++c) {
+        double a1[D], a2[D], a3[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            a1[i] = snap_at(A, D, 0, i, c, t);
+            a2[i] = snap_at(A, D, 1, i, c, t);
+            a3[i] = snap_at(A, D, 2, i, c, t);
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                if (i == j) continue;
+                if (i == 0 && j == 1 && c == D - 1) continue;
+                double d1 = a1[j], d2 = a2[j], d3 = a3[j];
+                if (fabs(d1) < DEN_MIN || fabs(d2) < DEN_MIN || fabs(d3) < DEN_MIN) continue;
+                double r1 = a1[i] / d1, r2 = a2[i] / d2, r3 = a3[i] / d3;
+                if (!isfinite(r1) || !isfinite(r2) || !isfinite(r3)) continue;
+                double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+                double den = r1 - 2.0 * r2 + r3;
+                if (fabs(den) > 1e-14 * sc) {
+                    double dd = r2 - r3;
+                    double L = r3 - dd * dd / den;
+                    if (isfinite(L) && fabs(L) > 1e-15) T.push(L, true);
+                }
+                if (fabs(r3) > 1e-15) T.push(r3, false);
+            }
+        }
+    }
+    const double nan = __builtin_nan("");
+    double limit = T.has_aitken ? T.first_aitken : (T.n > 0 ? T.first : nan);
+    const double ls2 = A.snap_ls[1 * A.sld + t];
+    const double ls3 = A.snap_ls[2 * A.sld + t];
+
+    // compute_precision
+    double prec;
+    if (fabs(q3) < 1e-280 || fabs(q2) < 1e-280) {
+        prec = nan;
+    } else {
+        double a = p3 / q3, b = p2 / q2;
+        if (!isfinite(a) || !isfinite(b)) {
+            prec = nan;
+        } else {
+            double diff = fabs(a - b);
+            prec = (diff <= 0.0) ? __builtin_inf() : -log10(diff);
+        }
+    }
+
+    // compute_kamidelta -> compute_delta
+    double delta = nan;
+    if (!(fabs(q3) < 1e-280 || fabs(q2) < 1e-280 || fabs(q1) < 1e-280)) {
+        double LN = p3 / q3, L2 = p2 / q2, L1 = p1 / q1;
+        if (isfinite(LN) && isfinite(L2) && isfinite(L1)) {
+            double La = nan;
+            double den = L1 - 2.0 * L2 + LN;
+            if (fabs(den) > 1e-30) {
+                double dd = L2 - LN;
+                La = LN - dd * dd / den;
+            }
+            double L = isfinite(limit) ? limit : (isfinite(La) ? La : LN);
+            // compute_delta(snaps, L)
+            if (!(fabs(q2) < 1e-280)) {
+                double log_q = log(fabs(q2)) + ls2;
+                if (log_q > 0.0) {
+                    double LNd = (fabs(q3) > 1e-280) ? p3 / q3 : nan;
+                    double Lu = L;
+                    if (!isfinite(Lu)) {
+                        if (fabs(q2) > 1e-280 && fabs(q1) > 1e-280) {
+                            double r1 = p1 / q1, r2 = p2 / q2, r3 = LNd;
+                            double dn = r1 - 2.0 * r2 + r3;
+                            if (isfinite(dn) && fabs(dn) > 1e-30) {
+                                double dd = r2 - r3;
+                                Lu = r3 - dd * dd / dn;
+                            }
+                        }
+                        if (!isfinite(Lu)) Lu = LNd;
+                    }
+                    if (isfinite(Lu)) {
+                        double approx = p2 / q2;
+                        if (isfinite(approx)) {
+                            double diff = fabs(Lu - approx);
+                            if (diff <= 0.0) {
+                                delta = __builtin_inf();
+                            } else if (isfinite(diff)) {
+                                delta = -(1.0 + log(diff) / log_q);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (A.valid) A.valid[t] = 1;
+    if (A.n_limits) A.n_limits[t] = T.n;
+    if (A.limit) A.limit[t] = limit;
+    if (A.rp) A.rp[t] = p3;
+    if (A.rq) A.rq[t] = q3;
+    if (A.log_scale) A.log_scale[t] = ls3;
+    if (A.precision) A.precision[t] = prec;
+    if (A.delta) A.delta[t] = delta;
+}
+
+// --------------------------------------------------------------------------
+// theta-companion column in the reference's operation order
+// (theta_companion_column + poly_theta_plus, cuda/cmf_pfq.cuh:127-181).
+// xv/yv hold the evaluation point's x_i and (y_i - 1) values.  Must be
+// compiled without FMA contraction: its bits feed the RNS residues.
+// --------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ bool companion_column_ref(const double (&xv)[8], const double (&yv)[8],
+                                                     int p, int q, double z, double (&col)[D]) {
+#pragma clang fp contract(off)
+    double px[9], py[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        px[k] = 0.0;
+        py[k] = 0.0;
+    }
+    px[0] = 1.0;
+    py[0] = 1.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < p) {
+            double acc[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k <= j; ++k) {
+                double pr = px[k] * xv[j];
+                acc[k] = acc[k] + pr;
+                acc[k + 1] = acc[k + 1] + px[k];
+            }
+#pragma unroll
+            for (int k = 0; k <= j + 1; ++k) px[k] = acc[k];
+        }
+        if (j < q) {
+            double acc[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k <= j; ++k) {
+                double pr = py[k] * yv[j];
+                acc[k] = acc[k] + pr;
+                acc[k + 1] = acc[k + 1] + py[k];
+            }
+#pragma unroll
+            for (int k = 0; k <= j + 1; ++k) py[k] = acc[k];
+        }
+    }
+    double dc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) dc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k <= 8; ++k)
+        if (k <= p) dc[k] = -z * px[k];
+#pragma unroll
+    for (int k = 0; k <= 8; ++k)
+        if (k <= q) dc[k + 1] = dc[k + 1] + py[k];
+    const double lead = dc[D];
+    double lscale = 1.0;
+    if (D == p && p == q + 1) {
+        lscale = fmax(1.0, fabs(1.0 - z));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < p) lscale += fabs(xv[i]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < q) lscale += fabs(yv[i]);
+    }
+    if (!isfinite(lead) || fabs(lead) <= 64.0 * kEps * lscale) return false;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        col[i] = -dc[i] / lead;
+        ok = ok && isfinite(col[i]);
+    }
+    return ok;
+}
+
+// Evaluation point and denominator of one axis step (apply_axis_step,
+// cuda/cmf_pfq.cuh:255-291).  ax is uniform; xs/ys are the current x and y
+// positions.  Returns inverse-ness; writes xv (x at eval), yv (y-1 at eval).
+__device__ __forceinline__ bool axis_eval(const double (&xs)[8], const double (&ys)[8], int p,
+                                          int ax, int sign, double (&xv)[8], double (&yv)[8],
+                                          double &denom) {
+#pragma clang fp contract(off)
+    const bool isx = ax < p;
+    const int ix = isx ? ax : -1;
+    const int iy = isx ? -1 : ax - p;
+    bool inverse;
+    if (isx) {
+        inverse = sign < 0;
+        double v = pick(xs, ix);
+        denom = inverse ? v - 1.0 : v;
+    } else {
+        inverse = sign > 0;
+        double v = pick(ys, iy);
+        denom = inverse ? v : v - 1.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        double ex = xs[j];
+        if (j == ix && inverse) ex = xs[j] - 1.0;
+        xv[j] = ex;
+        double ey = ys[j];
+        if (j == iy && inverse) ey = ys[j] + 1.0;
+        yv[j] = ey - 1.0;
+    }
+    return inverse;
+}
+
+__device__ __forceinline__ void axis_move(double (&xs)[8], double (&ys)[8], int p, int ax,
+                                          int sign) {
+#pragma clang fp contract(off)
+    const double s = (double)sign;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (ax < p && j == ax) xs[j] = xs[j] + s;
+        if (ax >= p && j == ax - p) ys[j] = ys[j] + s;
+    }
+}
+
+// Loads a trajectory's direction and start into registers.
+__device__ __forceinline__ void load_traj(const WalkArgs &A, int t, int32_t (&dir)[16],
+                                          double (&xs)[8], double (&ys)[8]) {
+#pragma unroll
+    for (int a = 0; a < 16; ++a) dir[a] = (a < A.naxes) ? A.dir[(size_t)a * A.ld + t] : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        xs[j] = (j < A.p) ? A.start[(size_t)j * A.ld + t] : 0.0;
+        ys[j] = (j < A.q) ? A.start[(size_t)(A.p + j) * A.ld + t] : 0.0;
+    }
+}
+
+}  // namespace cmfw

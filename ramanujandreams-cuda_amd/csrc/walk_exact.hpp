@@ -1,0 +1,154 @@
+// walk_exact.hpp -- the f64 walk in the reference's operation order.
+//
+// Every double operation rounds exactly where forward_walk /
+// right_mul_affine_companion / normalize_projective round
+// (cuda/cmf_pfq.cuh:100-371): no FMA contraction (each function body opts
+// out of it), correctly rounded division, the same summation order.  Used
+// by the reference-order kernel and as the fast kernel's path for inputs
+// outside its domain (non-integer or non-finite start points).
+#pragma once
+#include "walk_common.hpp"
+
+namespace cmfw {
+
+
+// right_mul_affine_companion (cuda/cmf_pfq.cuh:186-239), in place.
+template <int D>
+__device__ __forceinline__ bool right_mul_ref(double (&M)[D][D], const double (&col)[D],
+                                              double denom, bool inverse, double &ls) {
+#pragma clang fp contract(off)
+    if (!isfinite(denom) || fabs(denom) <= 64.0 * kEps) return false;
+    const double ia = 1.0 / denom;
+    bool ok = true;
+    if (!inverse) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            double dot = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) dot = dot + M[r][k] * col[k];
+#pragma unroll
+            for (int j = 0; j + 1 < D; ++j) M[r][j] = M[r][j] + M[r][j + 1] * ia;
+            M[r][D - 1] = M[r][D - 1] + dot * ia;
+        }
+    } else {
+        // beta, dot_beta and the solve denominator do not depend on the row:
+        // evaluate them once, with the reference's operations.
+        double be[D];
+        be[D - 1] = 1.0;
+#pragma unroll
+        for (int j = D - 2; j >= 0; --j) be[j] = -be[j + 1] * ia;
+        double db = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) db = db + be[k] * col[k];
+        const double sd = 1.0 + db * ia;
+        const double ss = fmax(1.0, fabs(db * ia));
+        if (!isfinite(sd) || fabs(sd) <= 128.0 * kEps * ss) return false;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            double al[D];
+            al[D - 1] = 0.0;
+#pragma unroll
+            for (int j = D - 2; j >= 0; --j) al[j] = M[r][j] - al[j + 1] * ia;
+            double da = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) da = da + al[k] * col[k];
+            const double tt = (M[r][D - 1] - da * ia) / sd;
+            ok = ok && isfinite(tt);
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                M[r][j] = al[j] + be[j] * tt;
+                ok = ok && isfinite(M[r][j]);
+            }
+        }
+        if (!ok) return false;
+    }
+    // normalize_projective (cuda/cmf_pfq.cuh:100-116); NaN entries are
+    // skipped by the max, as in the reference.
+    double mx = 0.0;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double v = fabs(M[r][c]);
+            mx = (v > mx) ? v : mx;
+        }
+    if (!(mx > 0.0) || !isfinite(mx)) return false;
+    const double inv = 1.0 / mx;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) M[r][c] = M[r][c] * inv;
+    ls = ls + log(mx);
+    return true;
+}
+
+// The whole walk of trajectory t (forward_walk, cuda/cmf_pfq.cuh:348-371)
+// followed by the snapshot reductions; in prefix mode only A.prefix steps.
+template <int D>
+__device__ void walk_exact_traj(const WalkArgs &A, int t) {
+#pragma clang fp contract(off)
+    int32_t dir[16];
+    double xs[8], ys[8];
+    load_traj(A, t, dir, xs, ys);
+    double M[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? 1.0 : 0.0;
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    const bool prefix = A.prefix > 0;
+    const int N = prefix ? A.prefix : A.N, h = A.h;
+    bool ok = prefix || !(N <= 0 || h <= 0 || N - 2 * h <= 0);
+    const int cp1 = prefix ? -1 : N - 2 * h, cp2 = prefix ? -1 : N - h;
+    const int cpN = prefix ? -1 : N;
+    double ls = 0.0;
+    int nf = 0, ni = 0;
+    for (int step = 1; ok && step <= N; ++step) {
+        // apply_trajectory_step (cuda/cmf_pfq.cuh:305-322)
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                double xv[8], yv[8], denom;
+                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                double col[D];
+                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col) &&
+                     right_mul_ref<D>(M, col, denom, inverse, ls);
+                if (!ok) break;
+                axis_move(xs, ys, A.p, ax, sign);
+            }
+        }
+        if (!ok) break;
+        const int s = (step == cp1) ? 0 : (step == cp2) ? 1 : (step == cpN) ? 2 : -1;
+        if (s >= 0) {
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    A.snap[((size_t)((s * D + i) * D + j)) * A.sld + t] = M[i][j];
+            A.snap_ls[(size_t)s * A.sld + t] = ls;
+        }
+    }
+    if (A.n_fwd) A.n_fwd[t] = nf;
+    if (A.n_inv) A.n_inv[t] = ni;
+    if (prefix) {
+        if (A.valid) A.valid[t] = ok ? 1 : 0;
+        if (A.log_scale) A.log_scale[t] = ok ? ls : __builtin_nan("");
+        return;
+    }
+    finish_from_snapshots<D>(A, t, ok);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_walk_f64_ref(WalkArgs A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.n) return;
+    walk_exact_traj<D>(A, t);
+}
+
+}  // namespace cmfw

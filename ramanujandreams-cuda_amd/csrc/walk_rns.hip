@@ -1,0 +1,324 @@
+// walk_rns.hip -- RNS CMF walk (gfx950), residues bit-exact with the
+// reference's integer arithmetic (cuda/cmf_walk_rns.cu:26-320).
+//
+// Lane mapping: a trajectory owns a group of G lanes (G = the power of two
+// >= n_primes); lane k of the group carries modulus k and its D x D residue
+// matrix in VGPRs.  All lanes of a group follow the same control flow (the
+// column and every failure test are group-uniform), so a wave64 walks
+// 64/G trajectories in lock-step per group.  The companion column is
+// evaluated in double by every lane in the reference's operation order and
+// rounded per lane (to_mod(llround(.))).  The 2-prime Garner CRT reads
+// lanes 0 and 1 of the group through cross-lane shuffles.
+//
+// Modular products use a double-precision quotient estimate: for a, b < m <
+// 2^30, q' = trunc(a*b*(1/m)) is within 1 of floor(a*b/m), so the low 32
+// bits of a*b - q'*m are the exact residue up to one +/-m correction.
+// Inverses are the reference's Fermat powers a^(m-2) (not inverses on the
+// composite moduli 5..7 of the default 8, kept for bit-exactness).  The
+// solve denominator of an inverse step does not depend on the row, so two
+// powers per lane per inverse step suffice where the reference takes d+1.
+#include "walk_common.hpp"
+#include "walk_kernels.hpp"
+
+namespace cmfw {
+
+__constant__ uint32_t kModuli[64] = {
+    1073741789u, 1073741783u, 1073741741u, 1073741723u, 1073741717u, 1073741707u,
+    1073741693u, 1073741683u, 1073741669u, 1073741657u, 1073741641u, 1073741623u,
+    1073741611u, 1073741597u, 1073741587u, 1073741573u, 1073741563u, 1073741551u,
+    1073741537u, 1073741527u, 1073741513u, 1073741503u, 1073741491u, 1073741477u,
+    1073741467u, 1073741453u, 1073741441u, 1073741427u, 1073741413u, 1073741403u,
+    1073741391u, 1073741377u, 1073741363u, 1073741353u, 1073741341u, 1073741329u,
+    1073741317u, 1073741307u, 1073741293u, 1073741281u, 1073741267u, 1073741257u,
+    1073741243u, 1073741231u, 1073741219u, 1073741207u, 1073741193u, 1073741183u,
+    1073741171u, 1073741157u, 1073741147u, 1073741137u, 1073741123u, 1073741111u,
+    1073741097u, 1073741087u, 1073741077u, 1073741063u, 1073741051u, 1073741039u,
+    1073741027u, 1073741017u, 1073741003u, 1073740991u};
+
+struct Modulus {
+    uint32_t m;
+    double inv;  // 1/m rounded
+};
+
+__device__ __forceinline__ uint32_t mulm(uint32_t a, uint32_t b, const Modulus &M) {
+    const double qd = (double)a * (double)b * M.inv;
+    const uint32_t qe = (uint32_t)qd;  // floor(ab/m) + {-1, 0, 1}
+    int32_t r = (int32_t)(a * b - qe * M.m);
+    r = (r < 0) ? r + (int32_t)M.m : r;
+    r = (r >= (int32_t)M.m) ? r - (int32_t)M.m : r;
+    return (uint32_t)r;
+}
+__device__ __forceinline__ uint32_t addm(uint32_t a, uint32_t b, uint32_t m) {
+    const uint32_t r = a + b;  // < 2^31
+    return r >= m ? r - m : r;
+}
+__device__ __forceinline__ uint32_t subm(uint32_t a, uint32_t b, uint32_t m) {
+    return a >= b ? a - b : a + m - b;
+}
+// a^e mod m, e = m - 2 (mod_inv, cuda/cmf_walk_rns.cu:68-78)
+__device__ __forceinline__ uint32_t fermat(uint32_t a, const Modulus &M) {
+    uint32_t e = M.m - 2u;
+    uint32_t r = 1u, b = a;
+    while (e) {
+        const uint32_t rb = mulm(r, b, M);
+        r = (e & 1u) ? rb : r;
+        b = mulm(b, b, M);
+        e >>= 1;
+    }
+    return r;
+}
+
+// llround with the out-of-range policy of the restatement: NaN or |x| >=
+// 2^63 -> INT64_MIN; otherwise round half away from zero.
+__device__ __forceinline__ int64_t llround_policy(double x) {
+    if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0))
+        return (int64_t)0x8000000000000000ull;
+    return (int64_t)round(x);
+}
+// to_mod (cuda/cmf_walk_rns.cu:61-66)
+__device__ __forceinline__ uint32_t to_mod(double x, uint32_t m) {
+    const int64_t v = llround_policy(x);
+    if (v >= 0) return (uint32_t)((uint64_t)v % m);
+    const uint64_t neg = (uint64_t)0 - (uint64_t)v;
+    return (uint32_t)((m - (uint32_t)(neg % m)) % m);
+}
+
+template <int D>
+__device__ __forceinline__ bool right_mul_rns_lane(uint32_t (&R)[D][D], const uint32_t (&cr)[D],
+                                                   uint32_t dr, bool inverse, const Modulus &M,
+                                                   bool active, uint64_t group_bits) {
+    const uint32_t m = M.m;
+    const uint32_t ia = fermat(dr, M);
+    if (!inverse) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            uint32_t dot = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) dot = addm(dot, mulm(R[r][k], cr[k], M), m);
+#pragma unroll
+            for (int j = 0; j + 1 < D; ++j) R[r][j] = addm(R[r][j], mulm(R[r][j + 1], ia, M), m);
+            R[r][D - 1] = addm(R[r][D - 1], mulm(dot, ia, M), m);
+        }
+        return true;
+    }
+    uint32_t be[D];
+    be[D - 1] = 1u;
+#pragma unroll
+    for (int j = D - 2; j >= 0; --j) be[j] = subm(0u, mulm(be[j + 1], ia, M), m);
+    uint32_t db = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) db = addm(db, mulm(be[k], cr[k], M), m);
+    const uint32_t sd = addm(1u, mulm(db, ia, M), m);
+    // any active lane of the group with sd == 0 invalidates the trajectory
+    const uint64_t zero = __ballot(active && sd == 0u);
+    if (zero & group_bits) return false;
+    const uint32_t isd = fermat(sd, M);
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        uint32_t al[D];
+        al[D - 1] = 0u;
+#pragma unroll
+        for (int j = D - 2; j >= 0; --j) al[j] = subm(R[r][j], mulm(al[j + 1], ia, M), m);
+        uint32_t da = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) da = addm(da, mulm(al[k], cr[k], M), m);
+        const uint32_t tt = mulm(subm(R[r][D - 1], mulm(da, ia, M), m), isd, M);
+#pragma unroll
+        for (int j = 0; j < D; ++j) R[r][j] = addm(al[j], mulm(be[j], tt, M), m);
+    }
+    return true;
+}
+
+// Garner CRT of one entry on moduli 0 and 1, with the reference's uint64
+// wrap-around (cuda/cmf_walk_rns.cu:232-244).
+__device__ __forceinline__ double crt_entry(uint32_t r0, uint32_t r1, int np, uint64_t m0inv,
+                                            uint64_t &val) {
+    if (np < 2) {
+        val = r0;
+        return (double)r0;
+    }
+    const uint64_t m0 = 1073741789ull, m1 = 1073741783ull;
+    const uint64_t diff = (r1 >= r0) ? (uint64_t)(r1 - r0) : ((uint64_t)r1 + m1 - r0);
+    const uint64_t t = (diff * m0inv) % m1;
+    const uint64_t v = (uint64_t)r0 + m0 * t;
+    const uint64_t half = (m0 * m1) / 2;
+    val = v;
+    return (v > half) ? -(double)(m0 * m1 - v) : (double)v;
+}
+
+struct RnsArgs {
+    WalkArgs w;
+    int np, G;
+    uint64_t m0inv;     // m0^(m1-2) mod m1
+    uint32_t *resid;    // [np][D][D][n] or null
+    uint64_t *crt;      // [D][D][n] or null
+};
+
+template <int D>
+__global__ __launch_bounds__(256) void k_walk_rns(RnsArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gtid / G;
+    const int k = gtid % G;
+    const int lane = threadIdx.x & 63;
+    const int base = lane - k;  // group's first lane within the wave
+    const uint64_t group_bits = (G >= 64) ? ~0ull : (((1ull << G) - 1ull) << base);
+    // Groups never straddle a wave (G divides 64) and whole groups are
+    // in or out of range, so group-wide operations see every member lane.
+    if (t >= A.n) return;
+    const bool active = k < RA.np;
+    Modulus M;
+    M.m = kModuli[active ? k : 0];
+    M.inv = 1.0 / (double)M.m;
+
+    int32_t dir[16];
+    double xs[8], ys[8];
+    load_traj(A, t, dir, xs, ys);
+    uint32_t R[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) R[i][j] = (i == j) ? 1u : 0u;
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    bool ok = true;
+    double ls = 0.0;
+    int nf = 0, ni = 0;
+    // The reference has no N-2h guard here: a checkpoint outside 1..N is
+    // never taken and its snapshot is read as a zero matrix.
+    if (k == 0) {
+        for (int s = 0; s < 3; ++s) {
+            const int cp = (s == 0) ? cp1 : (s == 1) ? cp2 : N;
+            if (cp >= 1 && cp <= N) continue;
+            for (int e = 0; e < D * D; ++e) A.snap[((size_t)(s * D * D + e)) * A.sld + t] = 0.0;
+            A.snap_ls[(size_t)s * A.sld + t] = 0.0;
+        }
+    }
+    for (int step = 1; ok && step <= N; ++step) {
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                double xv[8], yv[8], denom;
+                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                double col[D];
+                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col);
+                ok = ok && isfinite(denom) && !(fabs(denom) <= 64.0 * kEps);
+                if (!ok) break;
+                uint32_t cr[D];
+#pragma unroll
+                for (int i = 0; i < D; ++i) cr[i] = to_mod(col[i], M.m);
+                const uint32_t dr = to_mod(denom, M.m);
+                ok = right_mul_rns_lane<D>(R, cr, dr, inverse, M, active, group_bits);
+                if (!ok) break;
+                axis_move(xs, ys, A.p, ax, sign);
+            }
+        }
+        if (!ok) break;
+        if (step % 100 == 0 || step == cp1 || step == cp2 || step == N) {
+            if (step == N && RA.resid && active) {
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        RA.resid[((size_t)((k * D + i) * D + j)) * A.n + t] = R[i][j];
+            }
+            // rns_to_double: Garner CRT on lanes 0/1, then normalize_projective
+            double wm[D][D];
+            double mx = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    const uint32_t r0 = __shfl(R[i][j], base, 64);
+                    const uint32_t r1 = __shfl(R[i][j], base + 1, 64);
+                    uint64_t v;
+                    wm[i][j] = crt_entry(r0, r1, RA.np, RA.m0inv, v);
+                    if (step == N && RA.crt && k == 0)
+                        RA.crt[((size_t)(i * D + j)) * A.n + t] = v;
+                    const double a = fabs(wm[i][j]);
+                    mx = (a > mx) ? a : mx;
+                }
+            if ((mx > 0.0) && isfinite(mx)) {
+                const double inv = 1.0 / mx;
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) wm[i][j] = wm[i][j] * inv;
+                ls = ls + log(mx);
+            }
+            // every checkpoint equal to this step takes the snapshot (with
+            // h == 0 all three are taken at step N, as in the reference)
+            for (int s = 0; s < 3; ++s) {
+                const int cp = (s == 0) ? cp1 : (s == 1) ? cp2 : N;
+                if (cp != step || k != 0) continue;
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        A.snap[((size_t)((s * D + i) * D + j)) * A.sld + t] = wm[i][j];
+                A.snap_ls[(size_t)s * A.sld + t] = ls;
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) R[i][j] = to_mod(wm[i][j], M.m);
+        }
+    }
+    if (k == 0) {
+        if (A.n_fwd) A.n_fwd[t] = nf;
+        if (A.n_inv) A.n_inv[t] = ni;
+        finish_from_snapshots<D>(A, t, ok);
+    }
+}
+
+template <int D>
+static hipError_t launch_rns_d(const RnsArgs &RA, hipStream_t st) {
+    const int bs = 256;
+    const long long threads = (long long)RA.w.n * RA.G;
+    const int grid = (int)((threads + bs - 1) / bs);
+    hipLaunchKernelGGL(k_walk_rns<D>, dim3(grid), dim3(bs), 0, st, RA);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, uint64_t *crt,
+                           hipStream_t st) {
+    RnsArgs RA;
+    RA.w = A;
+    RA.np = np;
+    int G = 1;
+    while (G < np) G <<= 1;
+    RA.G = G;
+    // m0^(m1-2) mod m1 for the Garner step (host-side constant)
+    {
+        const uint64_t m0 = 1073741789ull, m1 = 1073741783ull;
+        uint64_t r = 1, b = m0 % m1, e = m1 - 2;
+        while (e) {
+            if (e & 1) r = (r * b) % m1;
+            b = (b * b) % m1;
+            e >>= 1;
+        }
+        RA.m0inv = r;
+    }
+    RA.resid = resid;
+    RA.crt = crt;
+    switch (D) {
+        case 2: return launch_rns_d<2>(RA, st);
+        case 3: return launch_rns_d<3>(RA, st);
+        case 4: return launch_rns_d<4>(RA, st);
+        case 5: return launch_rns_d<5>(RA, st);
+        case 6: return launch_rns_d<6>(RA, st);
+        case 7: return launch_rns_d<7>(RA, st);
+        case 8: return launch_rns_d<8>(RA, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cmfw

@@ -1,0 +1,90 @@
+"""GPU parity of cmfw_walk_rns with the oracle's restatement of the reference
+RNS walk: residues at step N and the Garner CRT values are bit-exact; the
+normalised snapshots that follow from them (p, q, limit) are bit-identical;
+log_scale (libm log) is compared to 1e-13."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_bits(a, b):
+    return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def _close(a, b, rtol=0.0, atol=0.0):
+    both_nan = np.isnan(a) & np.isnan(b)
+    same_inf = np.isinf(a) & np.isinf(b) & (np.sign(a) == np.sign(b))
+    fin = np.isfinite(a) & np.isfinite(b)
+    return bool((both_nan | same_inf | (fin & (np.abs(a - b) <= atol + rtol * np.abs(b)))).all())
+
+
+def _run(engine, orc, p, q, z, D, S, N, h, n_primes):
+    g, resid, crt = engine.walk_rns(p, q, z, D, S, N, h, n_primes=n_primes, want_resid=True,
+                                    want_crt=True)
+    g = engine.results_to_numpy(g)
+    resid = resid.cpu().numpy().view(np.uint32)
+    crt = crt.cpu().numpy().view(np.uint64)
+    rank = orc.make_desc(p, q, z).rank
+    for t in range(D.shape[0]):
+        r, s, W, V = orc.walk_rns_one(p, q, z, D[t], S[t], N, h, n_primes)
+        tag = (p, q, z, t)
+        assert g["valid"][t] == r.valid, tag
+        if not r.valid:
+            assert np.isnan(g["limit"][t])
+            continue
+        assert np.array_equal(resid[:, :, :, t], W[:, :rank, :rank]), tag
+        assert np.array_equal(crt[:, :, t], V[:rank, :rank]), tag
+        assert g["n_limits"][t] == r.n_limits, tag
+        for f in ("p", "q", "limit"):
+            assert _same_bits(np.array([g[f][t]]), np.array([getattr(r, f)])), (tag, f)
+        assert _close(np.array([g["log_scale"][t]]), np.array([r.log_scale]), 1e-13, 1e-13)
+    return g
+
+
+@pytest.mark.parametrize("p,q,z", [(3, 2, 0.5), (4, 3, 0.5), (6, 5, 1.0), (8, 7, 1.0),
+                                   (5, 4, 0.5), (8, 7, 0.5)])
+def test_rns_bit_exact_random(engine, orc, p, q, z):
+    rng = np.random.default_rng(100 + 7 * p + int(4 * z))
+    n = 45
+    D = rng.integers(0, 3, (n, p + q))
+    D[:, 0] = np.maximum(D[:, 0], 1)
+    D[: n // 3, p:] = 0  # x-only directions survive the z=1 zeta start
+    S = rng.integers(1, 9, (n, p + q)).astype(float)
+    if z == 1.0:
+        S[: n // 2] = [1] * p + [2] * q
+    print(f"walk_rns {p}F{q} z={z}", flush=True)
+    g = _run(engine, orc, p, q, z, D, S, 230, 40, 8)  # CRT re-encodes at 100, 150, 190, 200
+    assert g["valid"].sum() >= 5
+
+
+@pytest.mark.parametrize("n_primes", [1, 2, 3, 16, 64])
+def test_rns_prime_counts(engine, orc, n_primes):
+    p, q, z = 4, 3, 0.5
+    rng = np.random.default_rng(n_primes)
+    n = 20
+    D = np.ones((n, p + q), int)
+    D[::2, 4:] = 0
+    S = rng.integers(2, 9, (n, p + q)).astype(float)
+    print(f"walk_rns n_primes={n_primes}", flush=True)
+    _run(engine, orc, p, q, z, D, S, 120, 10, n_primes)
+
+
+def test_rns_no_guard_cases(engine, orc):
+    """The RNS kernel has no N-2h guard (cuda/cmf_walk_rns.cu:276-298): with
+    h = 0 all snapshots are taken at N; with N < 2h the first is a zero
+    matrix."""
+    p, q, z = 3, 2, 0.5
+    D = np.ones((6, 5), int)
+    S = np.array([[2, 3, 4, 6, 8]] * 6, float) + np.arange(6)[:, None]
+    for N, h in [(60, 0), (60, 40), (30, 30)]:
+        print(f"walk_rns no-guard N={N} h={h}", flush=True)
+        _run(engine, orc, p, q, z, D, S, N, h, 8)
+
+
+def test_rns_zeta_start_y_first_dies(engine, orc):
+    p, q, z = 6, 5, 1.0
+    S = np.array([[1] * 6 + [2] * 5] * 2, float)
+    D = np.array([[1] * 6 + [0] * 5, [0] * 10 + [1]])
+    g = engine.results_to_numpy(engine.walk_rns(p, q, z, D, S, 50, 10))
+    assert list(g["valid"]) == [1, 0]
