@@ -9,7 +9,8 @@ already resident in HBM:
      trajectory that fails there fails the full walk the same way);
   2. compact the survivors and sort them by direction so the lanes of a wave
      follow the same axis sequence;
-  3. deep walk: plain f64, N=1000, h=50, limits and delta on device;
+  3. deep walk: plain f64 (the fast engine: depth segments, FMA, power-of-two
+     rescaling), N=1000, h=50, limits and delta on device;
   4. verify: the top 1% of candidates by delta walked again in RNS
      (8 moduli, N=1000) -- all survivors when fewer than 1% survive.
 value = candidates resolved per second.  Kernel times are HIP-event
@@ -46,7 +47,7 @@ def parse():
     ap.add_argument("--n-cand", type=int, default=4_000_000)
     ap.add_argument("--top-frac", type=float, default=0.01)
     ap.add_argument("--n-primes", type=int, default=8)
-    ap.add_argument("--cpu-sample", type=int, default=200_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -90,7 +91,7 @@ class Pipeline:
         dl = self.dir[:, live].contiguous()
         sl = self.start[:, live].contiguous()
         mark("compact")
-        res = cw.walk_f64(P, Q, Z, dl, sl, N_DEPTH, H)
+        res = cw.walk_f64_fast(P, Q, Z, dl, sl, N_DEPTH, H)
         mark("deep")
         delta = res["delta"]
         ok = (res["valid"] == 1) & torch.isfinite(delta)

@@ -91,9 +91,10 @@ const char *cmfw_last_error(void);
 const char *cmfw_version(void);
 /* desc from (p, q, z); CMFW_EINVAL when rank > 8 or p + q > 16 or p < 1. */
 int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out);
-/* Device bytes of temporary storage a walk of n_traj trajectories needs
- * (snapshot staging); pass that much as `workspace` to avoid hipMallocAsync. */
-size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj);
+/* Device bytes of temporary storage any walk entry point needs for n_traj
+ * trajectories of depth N (snapshot staging, depth-segment products); pass
+ * at least that much as `workspace` to avoid hipMallocAsync. */
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h);
 
 /* Plain f64 walk.  Replaces walk_kernel_f64 (cuda/cmf_walk_f64.cu:94-154):
  * forward_walk + compute_limits + limit selection + compute_precision +
@@ -103,6 +104,19 @@ size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj);
 int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
                   cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
                   size_t workspace_bytes, void *stream);
+
+/* Fast f64 walk: the recurrence and validity rules of cmfw_walk_f64 with
+ * the arithmetic engineered for the VALU: depth segments walked as
+ * independent lanes and multiplied in order, fused multiply-adds, exact
+ * power-of-two rescaling with integer exponent bookkeeping, one solve
+ * denominator per inverse step.  Validity decisions equal the reference's
+ * (borderline solve denominators are re-evaluated in its operation order);
+ * values agree to rounding.  Start points that are not integers below 2^30
+ * in magnitude are walked in the reference's order.  No counterpart in the
+ * reference beyond walk_kernel_f64 itself. */
+int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                       cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                       size_t workspace_bytes, void *stream);
 
 /* Prefix walk: the first K trajectory steps of the f64 walk above, with
  * the same validity rules, for screening (a trajectory that fails within K

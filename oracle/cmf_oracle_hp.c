@@ -1,0 +1,149 @@
+/*
+ * cmf_oracle_hp.c -- quad-precision (binary128, 113-bit significand) walk
+ * of the pFq CMF recurrence (TEST INFRASTRUCTURE ONLY).
+ *
+ * The accuracy reference for the f64 and two-float walks: the same
+ * recurrence as cuda/cmf_pfq.cuh:127-371 (theta-companion column, O(d^2)
+ * forward / inverse right-multiply, projective normalisation) evaluated in
+ * __float128, so its rounding error (~1e-34 per operation) is far below
+ * what the double and float-float walks are judged to (1e-12 / 1e-10).
+ * It does not reproduce the reference's validity decisions (its thresholds
+ * are scaled to double precision); tests use it only on trajectories the
+ * double walks find valid.  Pinned in tests against exact rational walks.
+ */
+#include <math.h>
+#include <quadmath.h>
+#include <stdint.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef __float128 R;
+
+static void poly(const R *v, int c, R *out) {
+    for (int k = 0; k <= c; ++k) out[k] = 0;
+    out[0] = 1;
+    for (int j = 0; j < c; ++j) {
+        for (int k = j + 1; k >= 1; --k) out[k] = out[k - 1] + out[k] * v[j];
+        out[0] = out[0] * v[j];
+    }
+}
+
+/* column of the theta-companion matrix; 0 when the lead vanishes */
+static int column(const R *pos, R z, int dim, int p, int q, R *col) {
+    R xv[16], yv[16], px[10] = {0}, py[10] = {0}, D[11] = {0};
+    for (int i = 0; i < p; ++i) xv[i] = pos[i];
+    for (int i = 0; i < q; ++i) yv[i] = pos[p + i] - 1;
+    poly(xv, p, px);
+    poly(yv, q, py);
+    for (int k = 0; k <= p; ++k) D[k] = -z * px[k];
+    for (int k = 0; k <= q; ++k) D[k + 1] += py[k];
+    R lead = D[dim];
+    if (lead == 0) return 0;
+    for (int i = 0; i < dim; ++i) col[i] = -D[i] / lead;
+    return 1;
+}
+
+static int right_mul(R *M, int d, const R *col, R den, int inverse, R *ls) {
+    if (den == 0) return 0;
+    R out[64];
+    R ia = 1 / den;
+    if (!inverse) {
+        for (int r = 0; r < d; ++r) {
+            R *row = M + r * 8, dot = 0;
+            for (int k = 0; k < d; ++k) dot += row[k] * col[k];
+            for (int j = 0; j + 1 < d; ++j) out[r * 8 + j] = row[j] + row[j + 1] * ia;
+            out[r * 8 + d - 1] = row[d - 1] + dot * ia;
+        }
+    } else {
+        R be[8];
+        be[d - 1] = 1;
+        for (int j = d - 2; j >= 0; --j) be[j] = -be[j + 1] * ia;
+        R db = 0;
+        for (int k = 0; k < d; ++k) db += be[k] * col[k];
+        R sd = 1 + db * ia;
+        if (sd == 0) return 0;
+        for (int r = 0; r < d; ++r) {
+            R *row = M + r * 8, al[8], da = 0;
+            al[d - 1] = 0;
+            for (int j = d - 2; j >= 0; --j) al[j] = row[j] - al[j + 1] * ia;
+            for (int k = 0; k < d; ++k) da += al[k] * col[k];
+            R t = (row[d - 1] - da * ia) / sd;
+            for (int j = 0; j < d; ++j) out[r * 8 + j] = al[j] + be[j] * t;
+        }
+    }
+    R mx = 0;
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c) {
+            R v = fabsq(out[r * 8 + c]);
+            if (v > mx) mx = v;
+        }
+    if (!(mx > 0)) return 0;
+    for (int r = 0; r < d; ++r)
+        for (int c = 0; c < d; ++c) M[r * 8 + c] = out[r * 8 + c] / mx;
+    *ls += logq(mx);
+    return 1;
+}
+
+/* Walk one trajectory for N steps, snapshots at N-2h, N-h, N.  Outputs per
+ * snapshot k: ratio[k] = M[0][d-1]/M[1][d-1] (the primary p/q),
+ * pq[k][2] = normalised p and q, ls[k] = log scale (doubles, rounded from
+ * quad).  Returns 1 when no step hit an exactly singular case. */
+int hp_walk(int p, int q, double z, int rank, const int32_t *dir, const double *start, int N,
+            int h, double *ratio3, double *pq6, double *ls3) {
+    R M[64], pos[16];
+    for (int i = 0; i < 64; ++i) M[i] = 0;
+    for (int i = 0; i < rank; ++i) M[i * 8 + i] = 1;
+    for (int a = 0; a < 16; ++a) pos[a] = start[a];
+    const int nax = p + q;
+    int top = 0;
+    for (int a = 0; a < nax; ++a) top = abs(dir[a]) > top ? abs(dir[a]) : top;
+    const int cp[3] = {N - 2 * h, N - h, N};
+    R ls = 0, Z = z;
+    for (int step = 1; step <= N; ++step) {
+        for (int lev = top; lev >= 1; --lev)
+            for (int a = nax - 1; a >= 0; --a) {
+                if (abs(dir[a]) < lev) continue;
+                int s = dir[a] > 0 ? 1 : -1;
+                R ev[16];
+                memcpy(ev, pos, sizeof(ev));
+                R den;
+                int inv;
+                if (a < p) {
+                    inv = s < 0;
+                    den = s > 0 ? pos[a] : pos[a] - 1;
+                    if (s < 0) ev[a] -= 1;
+                } else {
+                    inv = s > 0;
+                    den = s < 0 ? pos[a] - 1 : pos[a];
+                    if (s > 0) ev[a] += 1;
+                }
+                R col[8];
+                if (!column(ev, Z, rank, p, q, col)) return 0;
+                if (!right_mul(M, rank, col, den, inv, &ls)) return 0;
+                pos[a] += s;
+            }
+        for (int k = 0; k < 3; ++k)
+            if (step == cp[k]) {
+                R pp = M[rank - 1], qq = M[8 + rank - 1];
+                ratio3[k] = (double)(pp / qq);
+                pq6[2 * k] = (double)pp;
+                pq6[2 * k + 1] = (double)qq;
+                ls3[k] = (double)ls;
+            }
+    }
+    return 1;
+}
+
+void hp_walk_batch(int p, int q, double z, int rank, int n, const int32_t *dirs,
+                   const double *starts, int N, int h, double *ratio3, double *pq6, double *ls3,
+                   int32_t *ok, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int i = 0; i < n; ++i)
+        ok[i] = hp_walk(p, q, z, rank, dirs + (size_t)i * 16, starts + (size_t)i * 16, N, h,
+                        ratio3 + 3 * (size_t)i, pq6 + 6 * (size_t)i, ls3 + 3 * (size_t)i);
+}

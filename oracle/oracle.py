@@ -100,6 +100,10 @@ def lib():
         L.orc_crt_entry.argtypes = [C.c_uint32, C.c_uint32, C.c_int, P(C.c_uint64)]
         L.orc_legacy_inputs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         P(C.c_int32), P(C.c_double)]
+        L.hp_walk_batch.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                    P(C.c_int32), P(C.c_double), C.c_int, C.c_int,
+                                    P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_int32),
+                                    C.c_int]
         L.orc_mt64_seed.argtypes = [C.c_void_p, C.c_uint64]
         L.orc_mt64_next.restype = C.c_uint64
         L.orc_mt64_next.argtypes = [C.c_void_p]
@@ -234,6 +238,22 @@ def ref_walk_f64_snaps(p, q, z, dir_, start, N, h):
     sn = np.zeros(195)
     ok = ref().ref_walk_f64(p, q, z, _iptr(D), _dptr(S), N, h, _dptr(out6), _iptr(nl), _dptr(sn))
     return bool(ok), sn[:192].reshape(3, 8, 8), sn[192:195]
+
+
+def hp_walk_batch(p, q, z, dirs, starts, N, h, nthreads=0):
+    """Quad-precision walk (cmf_oracle_hp.c): per trajectory the primary
+    ratio p/q at N-2h, N-h, N, normalised p, q and log scale, rounded to
+    double; ok = 0 where a step is exactly singular."""
+    d = make_desc(p, q, z)
+    D, S = as_traj_major(dirs, starts)
+    n = D.shape[0]
+    ratio = np.zeros((n, 3))
+    pq = np.zeros((n, 3, 2))
+    ls = np.zeros((n, 3))
+    ok = np.zeros(n, np.int32)
+    lib().hp_walk_batch(p, q, z, d.rank, n, _iptr(D), _dptr(S), N, h, _dptr(ratio), _dptr(pq),
+                        _dptr(ls), _iptr(ok), nthreads)
+    return {"ratio": ratio, "pq": pq, "log_scale": ls, "ok": ok}
 
 
 def legacy_inputs(p, q, n, mode, use_zeta):

@@ -12,6 +12,10 @@
 //   --seed S         seed of the legacy generators (default 42)
 //   --clean-inputs   generate one record per trajectory instead of the
 //                    reference's stride-aliased layout
+//   --fast           f64: the fast engine (depth segments, FMA, power-of-two
+//                    rescaling): same validity, values to rounding.  The
+//                    default is the reference's operation order, whose JSON
+//                    records equal the CPU restatement's bit for bit.
 //
 // INPUT FILE: 8-byte magic "CMFWIN1\0", int32 p, q, n, 0, then int32
 // dir[n][16] and double start[n][16] (trajectory-major, axes x1..xp,
@@ -43,6 +47,7 @@ static const int kDefaultBlock = 256;
 #endif
 
 static bool is_f64() { return strcmp(kVariant, "f64") == 0; }
+static bool g_fast = false;
 static bool is_rns() { return strcmp(kVariant, "rns") == 0; }
 
 #define HIP_OK(x)                                                                        \
@@ -119,6 +124,7 @@ static int launch(const cmfw_desc &d, const cmfw_inputs &in, int N, int h, int n
 #if defined(CMFW_VARIANT_df64)
     return cmfw_walk_f32x2(&d, &in, N, h, &res, nullptr, ws, ws_bytes, nullptr);
 #else
+    if (g_fast) return cmfw_walk_f64_fast(&d, &in, N, h, &res, nullptr, ws, ws_bytes, nullptr);
     return cmfw_walk_f64(&d, &in, N, h, &res, nullptr, ws, ws_bytes, nullptr);
 #endif
 }
@@ -173,7 +179,7 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     R.res.log_scale = R.alloc<double>(n);
     R.res.valid = R.alloc<int32_t>(n);
     R.res.n_limits = R.alloc<int32_t>(n);
-    R.ws_bytes = cmfw_workspace_bytes(&desc, n);
+    R.ws_bytes = cmfw_workspace_bytes(&desc, n, N, h);
     R.ws = R.alloc<char>(R.ws_bytes);
 
     // warm-up on one trajectory, as the reference does before timing
@@ -335,6 +341,7 @@ int main(int argc, char **argv) {
         else if (arg == "--inputs") inputs_path = next();
         else if (arg == "--seed") seed = strtoull(next(), nullptr, 10);
         else if (arg == "--clean-inputs") clean = true;
+        else if (arg == "--fast") g_fast = true;
         else {
             fprintf(stderr, is_f64() ? "Unknown arg: %s\n" : "Unknown: %s\n", argv[i]);
             return 1;

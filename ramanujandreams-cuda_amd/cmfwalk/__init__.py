@@ -51,11 +51,14 @@ SIGNATURES = {
     "cmfw_last_error": (C.c_char_p, []),
     "cmfw_version": (C.c_char_p, []),
     "cmfw_make_desc": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.POINTER(Desc)]),
-    "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32]),
+    "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32, C.c_int32, C.c_int32]),
     "cmfw_modulus": (C.c_uint32, [C.c_int32]),
     "cmfw_walk_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                 C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p, C.c_size_t,
                                 C.c_void_p]),
+    "cmfw_walk_f64_fast": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
+                                     C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p,
+                                     C.c_size_t, C.c_void_p]),
     "cmfw_walk_prefix_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32,
                                        C.POINTER(Results), C.c_void_p]),
     "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
@@ -188,6 +191,27 @@ def walk_f64(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspace=No
                            C.byref(_results_struct(out)), C.byref(sn) if sn else None, ws, wsb,
                            _stream_ptr(stream)))
     return out
+
+
+def walk_f64_fast(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspace=None,
+                  stream=None):
+    """Fast f64 walk (depth segments, FMA, power-of-two rescaling); same
+    validity decisions as walk_f64, values to rounding."""
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    out = out if out is not None else alloc_results(n, dt.device)
+    ws, wsb = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel())
+    sn = _snap_struct(snapshots)
+    _check(L.cmfw_walk_f64_fast(C.byref(d), C.byref(_inputs_struct(dt, st)), N, h,
+                                C.byref(_results_struct(out)), C.byref(sn) if sn else None, ws,
+                                wsb, _stream_ptr(stream)))
+    return out
+
+
+def workspace_bytes(p, q, z, n, N, h):
+    return int(load_library().cmfw_workspace_bytes(C.byref(make_desc(p, q, z)), n, N, h))
 
 
 def walk_prefix_f64(p, q, z, dirs, starts, K, out=None, stream=None):

@@ -54,7 +54,45 @@ size_t snap_bytes(int D, int n) {
     // [3][D][D][n] matrices + [3][n] log scales, 256-byte aligned pieces
     size_t m = (size_t)3 * D * D * (size_t)n * sizeof(double);
     size_t l = (size_t)3 * (size_t)n * sizeof(double);
-    return ((m + 255) & ~(size_t)255) + ((l + 255) & ~(size_t)255);
+    return cmfw::align256(m) + cmfw::align256(l);
+}
+
+// Lanes that keep every SIMD of an MI355X busy: 256 CUs x 4 SIMDs x 64 lanes
+// x 4 waves.  Fewer trajectories than that are split along their depth.
+int64_t g_target_lanes = 262144;
+int g_min_segment_steps = 25;
+
+// Depth segments of a fast walk: multiples of a segment length L plus the
+// snapshot depths N-2h, N-h, N; b[0] = 0, b[nseg] = N.
+int plan_segments(int n, int N, int h, int *b) {
+    if (N <= 0 || h <= 0 || N - 2 * h <= 0) {
+        b[0] = 0;
+        b[1] = 0;
+        return 1;  // the walk is invalid; one empty segment
+    }
+    int64_t want = (g_target_lanes + n - 1) / (n > 0 ? n : 1);
+    if (want < 1) want = 1;
+    int L = (int)((N + want - 1) / want);
+    if (L < g_min_segment_steps) L = g_min_segment_steps;
+    for (;;) {
+        int cnt = 0;
+        b[cnt++] = 0;
+        const int cps[3] = {N - 2 * h, N - h, N};
+        int next = L, ci = 0;
+        while (true) {
+            int v;
+            if (ci < 3 && (next > N || cps[ci] <= next)) v = cps[ci++];
+            else v = next;
+            if (v > b[cnt - 1]) {
+                if (cnt > cmfw::kMaxSegments) break;
+                b[cnt++] = v;
+            }
+            if (v == next) next += L;
+            if (b[cnt - 1] == N) break;
+        }
+        if (cnt <= cmfw::kMaxSegments + 1 && b[cnt - 1] == N) return cnt - 1;
+        L *= 2;
+    }
 }
 
 // Scratch snapshot storage: the caller's snaps, the caller's workspace, or
@@ -165,9 +203,11 @@ int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out) {
     return check_desc(&d);
 }
 
-size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj) {
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h) {
     if (!desc || n_traj < 0) return 0;
-    return snap_bytes(desc->rank, n_traj);
+    int b[cmfw::kMaxSegments + 1];
+    const int nseg = plan_segments(n_traj, N, h, b);
+    return snap_bytes(desc->rank, n_traj) + cmfw::fast_segment_bytes(desc->rank, n_traj, nseg);
 }
 
 uint32_t cmfw_modulus(int32_t k) {
@@ -188,6 +228,43 @@ int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     fill_args(A, desc, in, N, h, out, S);
     hipError_t e = cmfw::launch_walk_f64_ref(desc->rank, A, st);
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_f64 launch");
+    return release(S, st, rc);
+}
+
+int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                       cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                       size_t workspace_bytes, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (in->n_traj == 0) return CMFW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int n = in->n_traj;
+    int b[cmfw::kMaxSegments + 1];
+    const int nseg = plan_segments(n, N, h, b);
+    const size_t seg_need = cmfw::fast_segment_bytes(desc->rank, n, nseg);
+    SnapStore S;
+    void *seg = nullptr, *owned = nullptr;
+    if (workspace) {
+        const size_t sb = (snaps && snaps->m && snaps->ls) ? 0 : snap_bytes(desc->rank, n);
+        if (workspace_bytes < sb + seg_need)
+            return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes,
+                        sb + seg_need);
+        if ((rc = get_snaps(desc->rank, n, snaps, workspace, sb, st, S))) return rc;
+        seg = (char *)workspace + sb;
+    } else {
+        if ((rc = get_snaps(desc->rank, n, snaps, nullptr, 0, st, S))) return rc;
+        hipError_t e = hipMallocAsync(&owned, seg_need, st);
+        if (e != hipSuccess) return release(S, st, hip_fail(e, "hipMallocAsync(segments)"));
+        seg = owned;
+    }
+    cmfw::WalkArgs A;
+    fill_args(A, desc, in, N, h, out, S);
+    hipError_t e = cmfw::launch_walk_f64_fast(desc->rank, A, b, nseg, seg, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_f64_fast launch");
+    if (owned) {
+        hipError_t e2 = hipFreeAsync(owned, st);
+        if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
+    }
     return release(S, st, rc);
 }
 

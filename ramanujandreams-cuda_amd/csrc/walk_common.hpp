@@ -17,6 +17,9 @@
 namespace cmfw {
 
 constexpr double kEps = 2.220446049250313e-16;
+constexpr int kMaxSegments = 64;
+
+__host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct WalkArgs {
     int p, q, naxes, n, ld;

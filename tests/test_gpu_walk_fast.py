@@ -1,0 +1,169 @@
+"""GPU parity of cmfw_walk_f64_fast (depth segments, FMA, power-of-two
+rescaling) with the oracle.
+
+Validity decisions, executed-step counts and limit presence must be
+identical; values must be at least as accurate as the reference order's,
+judged against the quad-precision walk of the oracle.  Random signed
+directions give numerically chaotic walks (the reference's own error
+reaches 1e1 on some), so these batches compare the error distributions'
+median and maximum; tight accuracy bounds on well-conditioned
+configurations are in test_gpu_accuracy.py."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def check(g, r, tag, hp=None):
+    """Validity, executed-step counts and limit presence must equal the
+    reference order's.  Values: the primary ratio p/q at depth N must be no
+    less accurate than the reference order's, both measured against the
+    quad-precision walk (f64 random walks can be ill-conditioned: the
+    reference's own error reaches 1e-5 and worse on some)."""
+    assert (g["valid"] == r["valid"]).all(), tag
+    assert (g["n_fwd"] == r["n_fwd"]).all() and (g["n_inv"] == r["n_inv"]).all(), tag
+    v = r["valid"] == 1
+    if not v.any():
+        return
+    # n_limits counts ratios of every matrix entry whose denominator passes
+    # |d| >= 1e-280: an entry that cancels to exactly 0.0 in the reference's
+    # order can be a 1e-17 residue with FMA, so the count is rounding
+    # dependent.  Whether a walk has any limit (n_with_limits) must agree.
+    has_g, has_r = g["n_limits"][v] > 0, r["n_limits"][v] > 0
+    assert (has_g == has_r).mean() >= 0.98, (tag, "has-limit")
+    if hp is None:
+        return
+    v &= hp["ok"] == 1
+    exact = hp["ratio"][:, 2]
+    e_g = _rel(g["p"] / g["q"], exact)[v]
+    e_r = _rel(r["p"] / r["q"], exact)[v]
+    # compare the error distributions (on ill-conditioned walks both errors
+    # are amplified rounding noise, so walk-by-walk ratios are random)
+    q_g = np.percentile(e_g, [50, 90, 100])
+    q_r = np.percentile(e_r, [50, 90, 100])
+    print(f"{tag}: {int(v.sum())} walks, p/q error vs quad (median/p90/max): fast "
+          f"{q_g[0]:.1e}/{q_g[1]:.1e}/{q_g[2]:.1e}, reference {q_r[0]:.1e}/{q_r[1]:.1e}/"
+          f"{q_r[2]:.1e}", flush=True)
+    assert q_g[0] <= 4.0 * q_r[0] + 1e-14, tag  # typical accuracy
+    assert q_g[2] <= 8.0 * q_r[2] + 1e-12, tag  # no catastrophic walk
+    lg = np.abs(g["log_scale"][v] - hp["log_scale"][v, 2]) / np.maximum(1.0, np.abs(hp["log_scale"][v, 2]))
+    lr = np.abs(r["log_scale"][v] - hp["log_scale"][v, 2]) / np.maximum(1.0, np.abs(hp["log_scale"][v, 2]))
+    print(f"{tag}: log_scale error (median/max): fast {np.median(lg):.1e}/{lg.max():.1e}, "
+          f"reference {np.median(lr):.1e}/{lr.max():.1e}", flush=True)
+    assert np.median(lg) <= 4.0 * np.median(lr) + 1e-12, (tag, "log_scale")
+    # depth segments are products started from the identity mid-walk; on
+    # an ill-conditioned walk such a product carries decaying components
+    # whose rounding the sequential walk never sees, so the magnitude
+    # (not the direction) of the final product can drift by ~1e-6
+    # relative.  log_scale enters delta only through log|q|.
+    assert lg.max() <= max(8.0 * lr.max(), 1e-5), (tag, "log_scale")
+
+
+@pytest.mark.parametrize("p,q,z", [(3, 2, 0.5), (4, 3, 0.5), (5, 4, 1.0), (6, 5, 1.0),
+                                   (6, 5, 0.5), (8, 7, 1.0), (8, 7, 0.5), (5, 2, 0.5)])
+def test_fast_matches_reference_order(engine, orc, p, q, z):
+    rng = np.random.default_rng(31 * p + int(8 * z))
+    n = 500
+    D = rng.integers(-2, 3, (n, p + q))
+    D[: n // 2] = np.abs(D[: n // 2])
+    D[n // 2: n // 2 + 50] = 0
+    D[n // 2: n // 2 + 50, 0] = 1
+    S = rng.integers(1, 13, (n, p + q)).astype(float)
+    N, h = 240, 30
+    print(f"fast {p}F{q} z={z}", flush=True)
+    g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
+    r = orc.walk_f64_batch(p, q, z, D, S, N, h)
+    e = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, N, h))
+    hp = orc.hp_walk_batch(p, q, z, D, S, N, h)
+    # executed-step counters come from the reference-order kernel (the
+    # oracle does not count); its other fields equal the oracle bit for bit
+    check(g, _with_counts(r, e), f"{p}F{q} z={z}", hp)
+
+
+def _with_counts(r, e):
+    out = np.zeros(r.shape, dtype=e.dtype)
+    for f in r.dtype.names:
+        out[f] = r[f]
+    out["n_fwd"] = e["n_fwd"]
+    out["n_inv"] = e["n_inv"]
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 3000])
+def test_fast_segment_plans(engine, orc, n):
+    """Different batch sizes give different depth-segment plans (many short
+    segments for few trajectories); results must not depend on the plan."""
+    p, q, z = 6, 5, 1.0
+    rng = np.random.default_rng(n)
+    D = rng.integers(0, 3, (n, p + q))
+    D[:, p:] = 0
+    D[:, 0] = np.maximum(D[:, 0], 1)
+    S = np.tile(np.array([1.0] * p + [2.0] * q), (n, 1))
+    N, h = 400, 50
+    print(f"plans n={n}", flush=True)
+    g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
+    e = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, N, h))
+    check(g, e, f"n={n}", orc.hp_walk_batch(p, q, z, D, S, N, h))
+
+
+def test_fast_zeta_records(engine, published):
+    for run in published:
+        if run["variant"] != "f64" or run["suite"] != "zeta" or run["mode"] != "limit":
+            continue
+        p, q = run["p"], run["q"]
+        D = np.array([[1] + [0] * (p + q - 1)] * 3)
+        S = np.array([[1] * p + [2] * q] * 3, float)
+        g = engine.results_to_numpy(engine.walk_f64_fast(p, q, 1.0, D, S, 1000, 50))
+        rec = run["records"]
+        for f in ("p", "q", "log_scale", "limit"):
+            a = rec[f][0]
+            assert abs(a - g[f][0]) <= 1e-9 * max(1, abs(a)), (run["cmf"], f, a, g[f][0])
+        assert g["n_limits"][0] == rec["n_limits"][0]
+
+
+def test_fast_legacy_inputs_take_reference_path(engine, orc, published):
+    """Legacy delta-mode inputs hold non-integer start bit patterns: those
+    trajectories are walked in reference order, so counts match exactly."""
+    for run in published:
+        if run["variant"] != "f64" or run["mode"] != "delta" or run["suite"] != "random":
+            continue
+        p, q, z = run["p"], run["q"], float(run["z"])
+        D, S = orc.legacy_inputs(p, q, run["n_traj"], "delta", False)
+        g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D[:, :p + q], S[:, :p + q],
+                                                         run["N"], run["h"]))
+        valid = g["valid"].astype(bool)
+        assert int(valid.sum()) == run["n_valid"], run["cmf"]
+        assert int((valid & ~np.isnan(g["limit"])).sum()) == run["n_with_limits"], run["cmf"]
+
+
+def test_fast_invalid_depths(engine):
+    p, q, z = 3, 2, 0.5
+    D = np.ones((4, 5), int)
+    S = np.array([[2, 3, 1, 5, 7]] * 4, float)
+    for N, h in [(100, 50), (0, 10), (10, 0)]:
+        g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
+        assert (g["valid"] == 0).all()
+
+
+def test_prefix_walk_matches_full_walk_validity(engine, orc):
+    """A trajectory that fails within K steps fails the full walk; one that
+    survives K steps has the same counters as the full walk truncated."""
+    p, q, z = 8, 7, 1.0
+    D, S = engine.synth_inputs(42, 20000, p, q, 1, 0, 0, 4)
+    D, S = D[:, :15], S[:, :15]
+    pre = engine.results_to_numpy(engine.walk_prefix_f64(p, q, z, D, S, 1))
+    full = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, 60, 10))
+    assert not (full["valid"].astype(bool) & ~pre["valid"].astype(bool)).any()
+    dead = pre["valid"] == 0
+    assert (pre["n_fwd"][dead] == full["n_fwd"][dead]).all()
+    assert (pre["n_inv"][dead] == full["n_inv"][dead]).all()
+    # the first axis step of these candidates is a y+ (inverse) unless max y < max x
+    ymax, xmax = D[:, 8:].max(1), D[:, :8].max(1)
+    first_dies = ymax >= xmax
+    assert (pre["valid"][first_dies] == 0).all()
+    assert (pre["n_inv"][first_dies] + pre["n_fwd"][first_dies] == 1).all()

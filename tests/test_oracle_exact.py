@@ -236,3 +236,21 @@ def test_legacy_inputs_aliasing_map(orc):
     D, S = orc.legacy_inputs(6, 5, 30, "delta", True)
     assert (D[10:] == 0).all()
     assert (D[:10, :11] != 0).any(axis=1).all()
+
+
+@pytest.mark.parametrize("p,q,z,d,s", CASES)
+def test_quad_walk_against_exact_product(orc, p, q, z, d, s):
+    """The quad-precision accuracy oracle against exact rationals: the
+    primary ratio p/q and the log scale of the normalised product."""
+    N, h = 12, 3
+    W, ok = orc.exact_walk(p, q, Fraction(z), d, s, N)
+    hp = orc.hp_walk_batch(p, q, z, [d], [s], N, h)
+    if not ok:
+        return
+    dim = orc.make_desc(p, q, z).rank
+    c = dim - 1
+    if W[1][c] != 0:
+        exact = W[0][c] / W[1][c]
+        assert abs(hp["ratio"][0][2] - float(exact)) <= 1e-15 * max(1.0, abs(float(exact)))
+    mx = max(abs(x) for row in W for x in row)
+    assert abs(hp["log_scale"][0][2] - orc.exact_log_abs(mx)) <= 1e-13 * max(1, abs(hp["log_scale"][0][2]))
