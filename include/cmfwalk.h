@@ -92,9 +92,11 @@ const char *cmfw_version(void);
 /* desc from (p, q, z); CMFW_EINVAL when rank > 8 or p + q > 16 or p < 1. */
 int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out);
 /* Device bytes of temporary storage any walk entry point needs for n_traj
- * trajectories of depth N (snapshot staging, depth-segment products); pass
- * at least that much as `workspace` to avoid hipMallocAsync. */
-size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h);
+ * trajectories of depth N with n_primes RNS moduli (snapshot staging,
+ * depth-segment products, the Fermat-power table); pass at least that much
+ * as `workspace` to avoid hipMallocAsync. */
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h,
+                            int32_t n_primes);
 
 /* Plain f64 walk.  Replaces walk_kernel_f64 (cuda/cmf_walk_f64.cu:94-154):
  * forward_walk + compute_limits + limit selection + compute_precision +
@@ -131,7 +133,10 @@ int cmfw_walk_prefix_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K
  * n_primes residue lanes over the reference's modulus table, 2-prime Garner
  * CRT every 100 steps and at N-2h, N-h, N, re-encode of the normalised
  * doubles, then the same limit/delta reductions.  Residues and CRT values
- * are bit-exact with the reference's integer arithmetic.
+ * are bit-exact with the reference's integer arithmetic.  The products
+ * between CRT points are walked as independent depth segments (exact: the
+ * arithmetic is modular); start points that are not integers below 2^30 in
+ * magnitude are walked sequentially.
  * resid_out (nullable, device): final residues [n_primes][rank][rank][n].
  * crt_out (nullable, device): CRT values of the last snapshot
  * [rank][rank][n] as uint64. */
@@ -151,6 +156,15 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
 int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t start_lo,
                       int32_t start_hi, int32_t dir_lo, int32_t dir_hi, int32_t *dir_out,
                       double *start_out);
+
+/* Process-wide tuning knobs (read when a walk is launched; not for
+ * concurrent use with running calls).  Keys:
+ *   "rns_segments"      1 (default) walk RNS depth segments in parallel,
+ *                       0 walk every trajectory sequentially;
+ *   "target_lanes"      lanes a fast f64 walk aims to fill (262144);
+ *   "min_segment_steps" shortest fast f64 depth segment (25).
+ * No counterpart in the reference. */
+int cmfw_set_tuning(const char *key, int64_t value);
 
 /* Modulus k of the reference table (cuda/cmf_walk_rns.cu:26-43), 0 if out
  * of range. */

@@ -179,7 +179,7 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     R.res.log_scale = R.alloc<double>(n);
     R.res.valid = R.alloc<int32_t>(n);
     R.res.n_limits = R.alloc<int32_t>(n);
-    R.ws_bytes = cmfw_workspace_bytes(&desc, n, N, h);
+    R.ws_bytes = cmfw_workspace_bytes(&desc, n, N, h, n_primes);
     R.ws = R.alloc<char>(R.ws_bytes);
 
     // warm-up on one trajectory, as the reference does before timing

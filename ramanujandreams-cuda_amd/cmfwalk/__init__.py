@@ -51,7 +51,9 @@ SIGNATURES = {
     "cmfw_last_error": (C.c_char_p, []),
     "cmfw_version": (C.c_char_p, []),
     "cmfw_make_desc": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.POINTER(Desc)]),
-    "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32, C.c_int32, C.c_int32]),
+    "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32]),
+    "cmfw_set_tuning": (C.c_int, [C.c_char_p, C.c_int64]),
     "cmfw_modulus": (C.c_uint32, [C.c_int32]),
     "cmfw_walk_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                 C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p, C.c_size_t,
@@ -210,8 +212,14 @@ def walk_f64_fast(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspa
     return out
 
 
-def workspace_bytes(p, q, z, n, N, h):
-    return int(load_library().cmfw_workspace_bytes(C.byref(make_desc(p, q, z)), n, N, h))
+def workspace_bytes(p, q, z, n, N, h, n_primes=8):
+    return int(load_library().cmfw_workspace_bytes(C.byref(make_desc(p, q, z)), n, N, h,
+                                                   n_primes))
+
+
+def set_tuning(key, value):
+    """Process-wide tuning knob of the library (see include/cmfwalk.h)."""
+    _check(load_library().cmfw_set_tuning(key.encode(), int(value)))
 
 
 def walk_prefix_f64(p, q, z, dirs, starts, K, out=None, stream=None):

@@ -61,6 +61,7 @@ size_t snap_bytes(int D, int n) {
 // x 4 waves.  Fewer trajectories than that are split along their depth.
 int64_t g_target_lanes = 262144;
 int g_min_segment_steps = 25;
+bool g_rns_segments = true;
 
 // Depth segments of a fast walk: multiples of a segment length L plus the
 // snapshot depths N-2h, N-h, N; b[0] = 0, b[nseg] = N.
@@ -203,11 +204,21 @@ int cmfw_make_desc(int32_t p, int32_t q, double z, cmfw_desc *out) {
     return check_desc(&d);
 }
 
-size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h) {
+size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, int32_t h,
+                            int32_t n_primes) {
     if (!desc || n_traj < 0) return 0;
     int b[cmfw::kMaxSegments + 1];
     const int nseg = plan_segments(n_traj, N, h, b);
-    return snap_bytes(desc->rank, n_traj) + cmfw::fast_segment_bytes(desc->rank, n_traj, nseg);
+    size_t need = cmfw::fast_segment_bytes(desc->rank, n_traj, nseg);
+    if (n_primes >= 1 && n_primes <= CMFW_MAX_PRIMES) {
+        int rb[cmfw::kMaxRnsSegments + 1];
+        const int rseg = cmfw::rns_plan(N, h, rb, cmfw::kMaxRnsSegments);
+        if (rseg >= 0) {
+            const size_t r = cmfw::rns_seg_bytes(desc->rank, n_traj, n_primes, rseg);
+            need = need > r ? need : r;
+        }
+    }
+    return snap_bytes(desc->rank, n_traj) + need;
 }
 
 uint32_t cmfw_modulus(int32_t k) {
@@ -329,14 +340,64 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
         return fail(CMFW_EINVAL, "n_primes %d outside 1..64", n_primes);
     if (in->n_traj == 0) return CMFW_OK;
     hipStream_t st = (hipStream_t)stream;
-    SnapStore S;
-    if ((rc = get_snaps(desc->rank, in->n_traj, snaps, workspace, workspace_bytes, st, S)))
-        return rc;
+    const int n = in->n_traj;
+    int rb[cmfw::kMaxRnsSegments + 1];
+    const int rseg = g_rns_segments ? cmfw::rns_plan(N, h, rb, cmfw::kMaxRnsSegments) : -1;
     cmfw::WalkArgs A;
+    SnapStore S;
+    if (rseg < 0) {  // depth beyond the segment table: the sequential kernel alone
+        if ((rc = get_snaps(desc->rank, n, snaps, workspace, workspace_bytes, st, S))) return rc;
+        fill_args(A, desc, in, N, h, out, S);
+        hipError_t e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, nullptr, st);
+        rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_rns launch");
+        return release(S, st, rc);
+    }
+    const size_t seg_need = cmfw::rns_seg_bytes(desc->rank, n, n_primes, rseg);
+    void *seg = nullptr, *owned = nullptr;
+    if (workspace) {
+        const size_t sb = (snaps && snaps->m && snaps->ls) ? 0 : snap_bytes(desc->rank, n);
+        if (workspace_bytes < sb + seg_need)
+            return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes,
+                        sb + seg_need);
+        if ((rc = get_snaps(desc->rank, n, snaps, workspace, sb, st, S))) return rc;
+        seg = (char *)workspace + sb;
+    } else {
+        if ((rc = get_snaps(desc->rank, n, snaps, nullptr, 0, st, S))) return rc;
+        hipError_t e = hipMallocAsync(&owned, seg_need, st);
+        if (e != hipSuccess) return release(S, st, hip_fail(e, "hipMallocAsync(rns segments)"));
+        seg = owned;
+    }
     fill_args(A, desc, in, N, h, out, S);
-    hipError_t e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, st);
+    int32_t *fallback = nullptr;
+    hipError_t e = cmfw::launch_walk_rns_seg(desc->rank, A, n_primes, rb, rseg, seg, resid_out,
+                                             crt_out, &fallback, st);
+    if (e == hipSuccess && rseg > 0)  // start points off the integer grid
+        e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, fallback, st);
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_rns launch");
+    if (owned) {
+        hipError_t e2 = hipFreeAsync(owned, st);
+        if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
+    }
     return release(S, st, rc);
+}
+
+int cmfw_set_tuning(const char *key, int64_t value) {
+    if (!key) return fail(CMFW_EINVAL, "key is NULL");
+    if (!strcmp(key, "rns_segments")) {
+        g_rns_segments = value != 0;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "target_lanes")) {
+        if (value < 1) return fail(CMFW_EINVAL, "target_lanes must be >= 1");
+        g_target_lanes = value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "min_segment_steps")) {
+        if (value < 1 || value > 1000000) return fail(CMFW_EINVAL, "min_segment_steps out of range");
+        g_min_segment_steps = (int)value;
+        return CMFW_OK;
+    }
+    return fail(CMFW_EINVAL, "unknown tuning key %s", key);
 }
 
 }  // extern "C"

@@ -152,6 +152,7 @@ struct RnsArgs {
     uint64_t m0inv;     // m0^(m1-2) mod m1
     uint32_t *resid;    // [np][D][D][n] or null
     uint64_t *crt;      // [D][D][n] or null
+    const int32_t *only;  // null, or [n]: walk only trajectories with only[t] != 0
 };
 
 template <int D>
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(256) void k_walk_rns(RnsArgs RA) {
     // Groups never straddle a wave (G divides 64) and whole groups are
     // in or out of range, so group-wide operations see every member lane.
     if (t >= A.n) return;
+    if (RA.only && RA.only[t] == 0) return;
     const bool active = k < RA.np;
     Modulus M;
     M.m = kModuli[active ? k : 0];
@@ -289,7 +291,7 @@ static hipError_t launch_rns_d(const RnsArgs &RA, hipStream_t st) {
 }
 
 hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, uint64_t *crt,
-                           hipStream_t st) {
+                           const int32_t *only, hipStream_t st) {
     RnsArgs RA;
     RA.w = A;
     RA.np = np;
@@ -309,6 +311,7 @@ hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, ui
     }
     RA.resid = resid;
     RA.crt = crt;
+    RA.only = only;
     switch (D) {
         case 2: return launch_rns_d<2>(RA, st);
         case 3: return launch_rns_d<3>(RA, st);

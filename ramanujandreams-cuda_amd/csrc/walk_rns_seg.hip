@@ -1,0 +1,443 @@
+// walk_rns_seg.hip -- depth-parallel RNS walk, bit-exact with the
+// reference's residue arithmetic (cuda/cmf_walk_rns.cu:101-320).
+//
+// Between two CRT points (every 100 steps and N-2h, N-h, N) the RNS walk is
+// a pure product of axis matrices mod m: a forward step right-multiplies by
+// I + C*ia, and the reference's inverse step (alpha/beta recurrences, t
+// scaled by the Fermat power of the solve denominator) is a linear map of
+// each row, i.e. a right-multiplication by a fixed matrix -- on composite
+// moduli too, where that matrix is not the true inverse.  Matrix products
+// mod m are associative, so the segment products P_s can be computed in
+// parallel from the identity, and
+//     W(b_s) = reencode(W(b_{s-1})) * P_s   (mod m, every lane)
+// reproduces the reference residue for residue.
+//
+// Kernel 1 (k_rns_segment): one G-lane group per (trajectory, segment),
+// lane k = modulus k, P_s in Montgomery form in VGPRs.
+// Kernel 2 (k_rns_combine): one G-lane group per trajectory multiplies the
+// segment products in order, does the 2-prime Garner CRT through cross-lane
+// shuffles, normalises, snapshots and re-encodes at every segment end.
+//
+// Fermat powers of the step denominators come from a per-modulus table for
+// integer denominators |a| <= kDenTab (filled once per call); the solve
+// denominator is row-independent, so an inverse step takes one power.
+#include "rns_arith.hpp"
+#include "walk_common.hpp"
+#include "walk_kernels.hpp"
+
+namespace cmfw {
+
+constexpr int kDenTab = 16384;  // table covers denominators -kDenTab..kDenTab
+
+struct RnsConsts {
+    MontMod mods[kMaxRnsModuli];
+};
+
+struct RnsSegArgs {
+    WalkArgs w;
+    int np, G;
+    int nseg;
+    int b[kMaxRnsSegments + 1];  // segment boundaries = CRT points
+    RnsConsts C;
+    const uint32_t *dtab;   // [np][2*kDenTab+1] Montgomery a^(m-2) of a mod m
+    uint32_t *segP;         // [nseg][np][D][D][n]
+    int32_t *segV;          // [nseg][n]: 1 ok, 0 failed, 2 not an integer point
+    int32_t *segF, *segI;   // [nseg][n] executed fwd / inv axis steps
+    uint64_t m0inv;         // m0^(m1-2) mod m1
+    uint32_t *resid;        // [np][D][D][n] or null
+    uint64_t *crt;          // [D][D][n] or null
+    int32_t *fallback;      // [n]: 1 where the sequential kernel must walk
+};
+
+__global__ void k_rns_dentab(RnsConsts C, int np, uint32_t *dtab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i > 2 * kDenTab || k >= np) return;
+    const MontMod M = C.mods[k];
+    const double a = (double)(i - kDenTab);
+    const uint32_t r = to_mod_fast(a, M);
+    dtab[(size_t)k * (2 * kDenTab + 1) + i] = mpow_m2(to_mont(r, M), M);
+}
+
+__device__ __forceinline__ bool integer_point(double v) {
+    return v == rint(v) && fabs(v) < 1073741824.0;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gtid / G;
+    const int k = gtid % G;
+    const int s = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    const int base = lane - k;
+    const uint64_t group_bits = (G >= 64) ? ~0ull : (((1ull << G) - 1ull) << base);
+    if (t >= A.n) return;  // whole groups leave together
+    const bool active = k < RA.np;
+    const MontMod M = RA.C.mods[active ? k : 0];
+    const size_t o = (size_t)s * A.n + t;
+
+    int32_t dir[16];
+    double xs[8], ys[8];
+    load_traj(A, t, dir, xs, ys);
+    bool clean = true;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) clean = clean && integer_point(xs[j]) && integer_point(ys[j]);
+    if (!clean) {
+        if (k == 0) {
+            RA.segV[o] = 2;
+            RA.segF[o] = 0;
+            RA.segI[o] = 0;
+        }
+        return;
+    }
+    const double b0 = (double)RA.b[s];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        xs[j] = fma(b0, (double)(j < A.p ? dir[j] : 0), xs[j]);
+        ys[j] = fma(b0, (double)(j < A.q ? pick(dir, A.p + j) : 0), ys[j]);
+    }
+    uint32_t R[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) R[i][j] = (i == j) ? M.r1 : 0u;
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    const uint32_t *dt = RA.dtab + (size_t)(active ? k : 0) * (2 * kDenTab + 1);
+    const uint32_t m = M.m;
+    bool ok = true;
+    int nf = 0, ni = 0;
+    const int steps = RA.b[s + 1] - RA.b[s];
+    for (int st = 0; ok && st < steps; ++st) {
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                double xv[8], yv[8], denom;
+                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                double col[D];
+                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col);
+                ok = ok && isfinite(denom) && !(fabs(denom) <= 64.0 * kEps);
+                if (!ok) break;
+                uint32_t cr[D];
+#pragma unroll
+                for (int i = 0; i < D; ++i) cr[i] = to_mont(to_mod_fast(col[i], M), M);
+                const uint32_t ia = (fabs(denom) <= (double)kDenTab)
+                                        ? dt[(int)denom + kDenTab]
+                                        : mpow_m2(to_mont(to_mod_fast(denom, M), M), M);
+                if (!inverse) {
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        uint64_t acc = 0;
+#pragma unroll
+                        for (int kk = 0; kk < D; ++kk) acc += (uint64_t)R[r][kk] * cr[kk];
+                        const uint32_t dot = redc(acc, M);
+#pragma unroll
+                        for (int j = 0; j + 1 < D; ++j) R[r][j] = mfma(R[r][j + 1], ia, R[r][j], M);
+                        R[r][D - 1] = mfma(dot, ia, R[r][D - 1], M);
+                    }
+                } else {
+                    uint32_t be[D];
+                    be[D - 1] = M.r1;
+#pragma unroll
+                    for (int j = D - 2; j >= 0; --j) be[j] = msub(0u, mmul(be[j + 1], ia, M), m);
+                    uint64_t accb = 0;
+#pragma unroll
+                    for (int kk = 0; kk < D; ++kk) accb += (uint64_t)be[kk] * cr[kk];
+                    const uint32_t db = redc(accb, M);
+                    const uint32_t sd = mfma(db, ia, M.r1, M);
+                    // the reference fails when sd == 0 in any lane
+                    const uint64_t zero = __ballot(active && sd == 0u);
+                    if (zero & group_bits) {
+                        ok = false;
+                        break;
+                    }
+                    const uint32_t isd = mpow_m2(sd, M);
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        uint32_t al[D];
+                        al[D - 1] = 0u;
+#pragma unroll
+                        for (int j = D - 2; j >= 0; --j)
+                            al[j] = msub(R[r][j], mmul(al[j + 1], ia, M), m);
+                        uint64_t acca = 0;
+#pragma unroll
+                        for (int kk = 0; kk < D - 1; ++kk) acca += (uint64_t)al[kk] * cr[kk];
+                        const uint32_t da = redc(acca, M);
+                        const uint32_t tt = mmul(msub(R[r][D - 1], mmul(da, ia, M), m), isd, M);
+#pragma unroll
+                        for (int j = 0; j < D; ++j) R[r][j] = mfma(be[j], tt, al[j], M);
+                    }
+                }
+                axis_move(xs, ys, A.p, ax, sign);
+            }
+        }
+    }
+    if (k == 0) {
+        RA.segV[o] = ok ? 1 : 0;
+        RA.segF[o] = nf;
+        RA.segI[o] = ni;
+    }
+    if (!ok || !active) return;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            RA.segP[((((size_t)s * RA.np + k) * D + i) * D + j) * A.n + t] = R[i][j];
+}
+
+// Garner CRT of one entry on moduli 0 and 1, with the reference's uint64
+// wrap-around (cuda/cmf_walk_rns.cu:232-244).
+__device__ __forceinline__ double crt_garner(uint32_t r0, uint32_t r1, int np, uint64_t m0inv,
+                                             uint64_t &val) {
+    if (np < 2) {
+        val = r0;
+        return (double)r0;
+    }
+    const uint64_t m0 = 1073741789ull, m1 = 1073741783ull;
+    const uint64_t diff = (r1 >= r0) ? (uint64_t)(r1 - r0) : ((uint64_t)r1 + m1 - r0);
+    const uint64_t tq = (diff * m0inv) % m1;
+    const uint64_t v = (uint64_t)r0 + m0 * tq;
+    const uint64_t half = (m0 * m1) / 2;
+    val = v;
+    return (v > half) ? -(double)(m0 * m1 - v) : (double)v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gtid / G;
+    const int k = gtid % G;
+    const int lane = threadIdx.x & 63;
+    const int base = lane - k;
+    if (t >= A.n) return;
+    const bool active = k < RA.np;
+    const MontMod M = RA.C.mods[active ? k : 0];
+    if (RA.nseg > 0 && RA.segV[t] == 2) {  // not an integer point: the sequential kernel walks it
+        if (k == 0) RA.fallback[t] = 1;
+        return;
+    }
+    if (k == 0) RA.fallback[t] = 0;
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    // a checkpoint outside 1..N is never taken: its snapshot reads as zeros
+    if (k == 0) {
+        for (int sn = 0; sn < 3; ++sn) {
+            const int cp = (sn == 0) ? cp1 : (sn == 1) ? cp2 : N;
+            if (cp >= 1 && cp <= N) continue;
+            for (int e = 0; e < D * D; ++e) A.snap[((size_t)(sn * D * D + e)) * A.sld + t] = 0.0;
+            A.snap_ls[(size_t)sn * A.sld + t] = 0.0;
+        }
+    }
+    uint32_t W[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) W[i][j] = (i == j) ? M.r1 : 0u;
+    bool ok = true;
+    int nf = 0, ni = 0;
+    double ls = 0.0;
+    for (int s = 0; ok && s < RA.nseg; ++s) {
+        const size_t o = (size_t)s * A.n + t;
+        nf += RA.segF[o];
+        ni += RA.segI[o];
+        if (RA.segV[o] != 1) {
+            ok = false;
+            break;
+        }
+        // W <- W * P_s (mod m), row by row
+        const uint32_t *P = RA.segP + (((size_t)s * RA.np + (active ? k : 0)) * D * D) * A.n + t;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            uint32_t row[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                uint64_t acc = 0;
+#pragma unroll
+                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * A.n];
+                row[j] = redc(acc, M);
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) W[r][j] = row[j];
+        }
+        // CRT point: rns_to_double, snapshot, re-encode (cuda/cmf_walk_rns.cu:285-296)
+        const int step = RA.b[s + 1];
+        uint32_t Wn[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) Wn[i][j] = from_mont(W[i][j], M);
+        if (step == N && RA.resid && active) {
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    RA.resid[((size_t)((k * D + i) * D + j)) * A.n + t] = Wn[i][j];
+        }
+        double wm[D][D];
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const uint32_t r0 = __shfl(Wn[i][j], base, 64);
+                const uint32_t r1 = __shfl(Wn[i][j], base + 1, 64);
+                uint64_t v;
+                wm[i][j] = crt_garner(r0, r1, RA.np, RA.m0inv, v);
+                if (step == N && RA.crt && k == 0) RA.crt[((size_t)(i * D + j)) * A.n + t] = v;
+                const double a = fabs(wm[i][j]);
+                mx = (a > mx) ? a : mx;
+            }
+        if ((mx > 0.0) && isfinite(mx)) {
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) wm[i][j] = wm[i][j] * inv;
+            ls = ls + log(mx);
+        }
+        for (int sn = 0; sn < 3; ++sn) {
+            const int cp = (sn == 0) ? cp1 : (sn == 1) ? cp2 : N;
+            if (cp != step || k != 0) continue;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    A.snap[((size_t)((sn * D + i) * D + j)) * A.sld + t] = wm[i][j];
+            A.snap_ls[(size_t)sn * A.sld + t] = ls;
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) W[i][j] = to_mont(to_mod_fast(wm[i][j], M), M);
+    }
+    if (k == 0) {
+        if (A.n_fwd) A.n_fwd[t] = nf;
+        if (A.n_inv) A.n_inv[t] = ni;
+        finish_from_snapshots<D>(A, t, ok);
+    }
+}
+
+template <int D>
+static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
+    const int bs = 256;
+    const long long threads = (long long)RA.w.n * RA.G;
+    const int gx = (int)((threads + bs - 1) / bs);
+    if (RA.nseg > 0) {
+        hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_rns_combine<D>, dim3(gx), dim3(bs), 0, st, RA);
+    return hipGetLastError();
+}
+
+// Host-side Montgomery constants of modulus m.
+static MontMod make_mont(uint32_t m) {
+    MontMod M;
+    M.m = m;
+    uint32_t inv = 1;  // m^-1 mod 2^32 by Newton iteration (m odd)
+    for (int i = 0; i < 5; ++i) inv *= 2u - m * inv;
+    M.mp = (uint32_t)(0u - inv);
+    const uint64_t r1 = (((uint64_t)1) << 32) % m;
+    M.r1 = (uint32_t)r1;
+    M.r2 = (uint32_t)((r1 * r1) % m);
+    M.inv = 1.0 / (double)m;
+    return M;
+}
+
+size_t rns_seg_bytes(int D, int n, int np, int nseg) {
+    const size_t nn = (size_t)n;
+    return align256((size_t)nseg * np * D * D * nn * 4) + 3 * align256((size_t)nseg * nn * 4) +
+           align256(nn * 4) + align256((size_t)np * (2 * kDenTab + 1) * 4);
+}
+
+// CRT points of an RNS walk of depth N: multiples of 100 and the
+// checkpoints N-2h, N-h, N inside 1..N (cuda/cmf_walk_rns.cu:285).  Returns
+// the segment count (0 when N <= 0: no step is walked), or -1 when more
+// than maxseg segments would be needed.
+int rns_plan(int N, int h, int *b, int maxseg) {
+    int cnt = 0;
+    b[cnt++] = 0;
+    if (N <= 0) return 0;
+    const int cps[3] = {N - 2 * h, N - h, N};
+    for (int step = 1; step <= N;) {
+        int nxt = ((step + 99) / 100) * 100;
+        for (int c = 0; c < 3; ++c)
+            if (cps[c] >= step && cps[c] < nxt) nxt = cps[c];
+        if (nxt > N) nxt = N;
+        if (cnt > maxseg) return -1;
+        b[cnt++] = nxt;
+        step = nxt + 1;
+    }
+    return cnt - 1;
+}
+
+hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds, int nseg,
+                               void *ws, uint32_t *resid, uint64_t *crt, int32_t **fallback,
+                               hipStream_t st) {
+    RnsSegArgs RA;
+    RA.w = A;
+    RA.np = np;
+    int G = 1;
+    while (G < np) G <<= 1;
+    RA.G = G;
+    RA.nseg = nseg;
+    for (int i = 0; i <= nseg; ++i) RA.b[i] = bounds[i];
+    for (int k = 0; k < np; ++k) RA.C.mods[k] = make_mont(host_modulus(k));
+    const size_t nn = (size_t)A.n;
+    char *p = (char *)ws;
+    RA.segP = (uint32_t *)p;
+    p += align256((size_t)nseg * np * D * D * nn * 4);
+    RA.segV = (int32_t *)p;
+    p += align256((size_t)nseg * nn * 4);
+    RA.segF = (int32_t *)p;
+    p += align256((size_t)nseg * nn * 4);
+    RA.segI = (int32_t *)p;
+    p += align256((size_t)nseg * nn * 4);
+    RA.fallback = (int32_t *)p;
+    p += align256(nn * 4);
+    uint32_t *dtab = (uint32_t *)p;
+    RA.dtab = dtab;
+    {
+        const int bs = 256;
+        dim3 grid((2 * kDenTab + 1 + bs - 1) / bs, np);
+        hipLaunchKernelGGL(k_rns_dentab, grid, dim3(bs), 0, st, RA.C, np, dtab);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    {
+        const uint64_t m0 = 1073741789ull, m1 = 1073741783ull;
+        uint64_t r = 1, bb = m0 % m1, e = m1 - 2;
+        while (e) {
+            if (e & 1) r = (r * bb) % m1;
+            bb = (bb * bb) % m1;
+            e >>= 1;
+        }
+        RA.m0inv = r;
+    }
+    RA.resid = resid;
+    RA.crt = crt;
+    *fallback = RA.fallback;
+    switch (D) {
+        case 2: return launch_rns_seg_d<2>(RA, st);
+        case 3: return launch_rns_seg_d<3>(RA, st);
+        case 4: return launch_rns_seg_d<4>(RA, st);
+        case 5: return launch_rns_seg_d<5>(RA, st);
+        case 6: return launch_rns_seg_d<6>(RA, st);
+        case 7: return launch_rns_seg_d<7>(RA, st);
+        case 8: return launch_rns_seg_d<8>(RA, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cmfw

@@ -45,7 +45,7 @@ def test_host_helpers():
     assert cmfwalk.modulus(0) == 1073741789 and cmfwalk.modulus(63) == 1073740991
     assert cmfwalk.modulus(64) == 0
     L = cmfwalk.load_library()
-    assert L.cmfw_workspace_bytes(cmfwalk.make_desc(8, 7, 0.5), 1000, 1000, 50) >= 3 * 64 * 1000 * 8
+    assert L.cmfw_workspace_bytes(cmfwalk.make_desc(8, 7, 0.5), 1000, 1000, 50, 8) >= 3 * 64 * 1000 * 8
 
 
 def test_moduli_agree_with_oracle(orc):

@@ -88,3 +88,57 @@ def test_rns_zeta_start_y_first_dies(engine, orc):
     D = np.array([[1] * 6 + [0] * 5, [0] * 10 + [1]])
     g = engine.results_to_numpy(engine.walk_rns(p, q, z, D, S, 50, 10))
     assert list(g["valid"]) == [1, 0]
+
+
+def _rns_all(engine, p, q, z, D, S, N, h, np_):
+    g, resid, crt = engine.walk_rns(p, q, z, D, S, N, h, n_primes=np_, want_resid=True,
+                                    want_crt=True)
+    return engine.results_to_numpy(g), resid.cpu().numpy(), crt.cpu().numpy()
+
+
+@pytest.mark.parametrize("p,q,z,N,h", [(8, 7, 1.0, 1000, 50), (4, 3, 0.5, 333, 40),
+                                       (6, 5, 0.5, 250, 0), (3, 2, 0.5, 90, 50)])
+def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h):
+    """Depth-segmented RNS (products between CRT points walked in parallel)
+    against the sequential kernel: residues, CRT values and every result
+    field identical."""
+    rng = np.random.default_rng(N + p)
+    n = 300
+    D = rng.integers(0, 3, (n, p + q))
+    D[:, 0] = np.maximum(D[:, 0], 1)
+    D[: n // 2, p:] = 0
+    S = rng.integers(1, 9, (n, p + q)).astype(float)
+    if z == 1.0:
+        S[:] = [1] * p + [2] * q
+    print(f"rns segments vs sequential {p}F{q} N={N} h={h}", flush=True)
+    a, ra, ca = _rns_all(engine, p, q, z, D, S, N, h, 8)
+    engine.set_tuning("rns_segments", 0)
+    try:
+        b, rb, cb = _rns_all(engine, p, q, z, D, S, N, h, 8)
+    finally:
+        engine.set_tuning("rns_segments", 1)
+    assert (a["valid"] == b["valid"]).all()
+    v = a["valid"] == 1
+    assert np.array_equal(ra[..., v], rb[..., v]) and np.array_equal(ca[..., v], cb[..., v])
+    for f in ("p", "q", "limit", "precision", "delta", "log_scale"):
+        assert _same_bits(a[f], b[f]), f
+    for f in ("n_limits", "n_fwd", "n_inv"):
+        assert (a[f] == b[f]).all(), f
+    assert v.sum() >= 10
+
+
+def test_rns_legacy_inputs_fallback(engine, orc, published):
+    """Legacy delta-mode starts are int bit patterns, not integer points:
+    the segmented walk hands them to the sequential kernel; published
+    validity counts of the RNS runs must reproduce."""
+    for run in published:
+        if run["variant"] != "rns" or run["suite"] != "random" or run["cmf"] not in ("3F2", "4F3"):
+            continue
+        p, q, z = run["p"], run["q"], float(run["z"])
+        D, S = orc.legacy_inputs(p, q, run["n_traj"], run["mode"], False)
+        print(f"rns published {run['cmf']} {run['mode']}", flush=True)
+        g = engine.results_to_numpy(engine.walk_rns(p, q, z, D[:, :p + q], S[:, :p + q],
+                                                    run["N"], run["h"], n_primes=run["n_primes"]))
+        valid = g["valid"].astype(bool)
+        assert int(valid.sum()) == run["n_valid"], (run["cmf"], run["mode"])
+        assert int((valid & ~np.isnan(g["limit"])).sum()) == run["n_with_limits"], run["cmf"]
