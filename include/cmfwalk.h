@@ -114,11 +114,26 @@ int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
  * denominator per inverse step.  Validity decisions equal the reference's
  * (borderline solve denominators are re-evaluated in its operation order);
  * values agree to rounding.  Start points that are not integers below 2^30
- * in magnitude are walked in the reference's order.  No counterpart in the
+ * in magnitude are walked in the reference's order (below 2^23: the
+ * integer grid shared with cmfw_walk_f32x2).  No counterpart in the
  * reference beyond walk_kernel_f64 itself. */
 int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
                        cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
                        size_t workspace_bytes, void *stream);
+
+/* Compensated two-float walk: the fast engine's structure with the state,
+ * the theta polynomials, the columns and every update in float-float FP32
+ * (TwoProd through fmaf, TwoSum renormalisation; ~48 significand bits).
+ * FP64 is used for the power-of-two exponent's log, the snapshot's
+ * max-normalisation, p/q, delta, and for re-deciding a borderline solve
+ * denominator in the reference's operation order (validity parity).
+ * Replaces walk_kernel_df64 (cuda/cmf_walk_df64.cu:304-349), whose
+ * float-float arithmetic with single-float reciprocals it does not
+ * reproduce.  Start points that are not integers below 2^23 in magnitude
+ * are walked in the reference's FP64 order. */
+int cmfw_walk_f32x2(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                    cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                    size_t workspace_bytes, void *stream);
 
 /* Prefix walk: the first K trajectory steps of the f64 walk above, with
  * the same validity rules, for screening (a trajectory that fails within K

@@ -61,6 +61,9 @@ SIGNATURES = {
     "cmfw_walk_f64_fast": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                      C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p,
                                      C.c_size_t, C.c_void_p]),
+    "cmfw_walk_f32x2": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
+                                  C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p,
+                                  C.c_size_t, C.c_void_p]),
     "cmfw_walk_prefix_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32,
                                        C.POINTER(Results), C.c_void_p]),
     "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
@@ -209,6 +212,22 @@ def walk_f64_fast(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspa
     _check(L.cmfw_walk_f64_fast(C.byref(d), C.byref(_inputs_struct(dt, st)), N, h,
                                 C.byref(_results_struct(out)), C.byref(sn) if sn else None, ws,
                                 wsb, _stream_ptr(stream)))
+    return out
+
+
+def walk_f32x2(p, q, z, dirs, starts, N, h, out=None, snapshots=None, workspace=None,
+               stream=None):
+    """Compensated two-float FP32 walk (the fast engine's structure)."""
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    out = out if out is not None else alloc_results(n, dt.device)
+    ws, wsb = (None, 0) if workspace is None else (workspace.data_ptr(), workspace.numel())
+    sn = _snap_struct(snapshots)
+    _check(L.cmfw_walk_f32x2(C.byref(d), C.byref(_inputs_struct(dt, st)), N, h,
+                             C.byref(_results_struct(out)), C.byref(sn) if sn else None, ws, wsb,
+                             _stream_ptr(stream)))
     return out
 
 

@@ -1,0 +1,97 @@
+// arith.hpp -- the two arithmetics of the fast walk engine.
+//
+// ArithF64: IEEE double with explicit fused multiply-adds.
+// ArithFF:  compensated two-float FP32 ("float-float", value = hi + lo,
+//           |lo| <= ulp(hi)/2): TwoProd through fmaf, TwoSum
+//           renormalisation.  About 48 significand bits at FP32 VALU
+//           instructions; its own error bound is ~2^-47 per operation.
+// Both expose the same small interface so walk_fast.hpp is written once.
+// The translation units are compiled with -ffp-contract=off: the
+// error-free transformations below must round exactly where written.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cmfw {
+
+struct ArithF64 {
+    using T = double;
+    using M = double;  // magnitude type for max / range checks
+    static constexpr const char *name = "f64";
+    static constexpr int kRangeLog2 = 400;  // keep max |M| in [2^-400, 2^400]
+    __device__ static T zero() { return 0.0; }
+    __device__ static T one() { return 1.0; }
+    __device__ static T from_d(double x) { return x; }
+    __device__ static double to_d(T x) { return x; }
+    __device__ static T add(T a, T b) { return a + b; }
+    __device__ static T mul(T a, T b) { return a * b; }
+    __device__ static T fma(T a, T b, T c) { return ::fma(a, b, c); }
+    __device__ static T neg(T a) { return -a; }
+    __device__ static T recip(T a) { return 1.0 / a; }
+    __device__ static M mag(T a) { return fabs(a); }
+    __device__ static T scale2(T a, int e) { return ldexp(a, e); }
+    __device__ static bool finite(T a) { return isfinite(a); }
+};
+
+struct ff {
+    float hi, lo;
+};
+
+struct ArithFF {
+    using T = ff;
+    using M = float;
+    static constexpr const char *name = "f32x2";
+    static constexpr int kRangeLog2 = 60;  // keep max |hi| in [2^-60, 2^60]
+    __device__ static T zero() { return {0.f, 0.f}; }
+    __device__ static T one() { return {1.f, 0.f}; }
+    __device__ static T from_d(double x) {
+        const float h = (float)x;
+        return {h, (float)(x - (double)h)};
+    }
+    __device__ static double to_d(T x) { return (double)x.hi + (double)x.lo; }
+    // |a| >= |b| or a == 0
+    __device__ static T fast2(float a, float b) {
+        const float s = a + b;
+        return {s, b - (s - a)};
+    }
+    __device__ static T two_sum(float a, float b) {
+        const float s = a + b;
+        const float bb = s - a;
+        return {s, (a - (s - bb)) + (b - bb)};
+    }
+    __device__ static T add(T a, T b) {
+        const T s = two_sum(a.hi, b.hi);
+        const T t = two_sum(a.lo, b.lo);
+        const T u = fast2(s.hi, s.lo + t.hi);
+        return fast2(u.hi, u.lo + t.lo);
+    }
+    __device__ static T mul(T a, T b) {
+        const float p = a.hi * b.hi;
+        float e = fmaf(a.hi, b.hi, -p);
+        e = fmaf(a.hi, b.lo, e);
+        e = fmaf(a.lo, b.hi, e);
+        return fast2(p, e);
+    }
+    // a*b + c with one renormalisation chain
+    __device__ static T fma(T a, T b, T c) {
+        const float p = a.hi * b.hi;
+        float e = fmaf(a.hi, b.hi, -p);
+        e = fmaf(a.hi, b.lo, e);
+        e = fmaf(a.lo, b.hi, e);
+        const T s = two_sum(p, c.hi);
+        const float l = s.lo + (e + c.lo);
+        return fast2(s.hi, l);
+    }
+    __device__ static T neg(T a) { return {-a.hi, -a.lo}; }
+    __device__ static T recip(T a) {
+        // r0 = 1/hi, one Newton step on the float-float value
+        const float r = 1.0f / a.hi;
+        const T ar = mul(a, {r, 0.f});          // ~ 1
+        const T e = add({1.f, 0.f}, neg(ar));   // 1 - a*r
+        return add({r, 0.f}, mul({r, 0.f}, e));
+    }
+    __device__ static M mag(T a) { return fabsf(a.hi); }
+    __device__ static T scale2(T a, int e) { return {ldexpf(a.hi, e), ldexpf(a.lo, e)}; }
+    __device__ static bool finite(T a) { return isfinite(a.hi) && isfinite(a.lo); }
+};
+
+}  // namespace cmfw

@@ -242,9 +242,10 @@ int cmfw_walk_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     return release(S, st, rc);
 }
 
-int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
-                       cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
-                       size_t workspace_bytes, void *stream) {
+// Shared host side of the fast engine (FP64 or two-float).
+static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *in, int32_t N,
+                            int32_t h, cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                            size_t workspace_bytes, void *stream) {
     int rc;
     if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
     if (in->n_traj == 0) return CMFW_OK;
@@ -270,13 +271,26 @@ int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, 
     }
     cmfw::WalkArgs A;
     fill_args(A, desc, in, N, h, out, S);
-    hipError_t e = cmfw::launch_walk_f64_fast(desc->rank, A, b, nseg, seg, st);
-    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_f64_fast launch");
+    hipError_t e = ff ? cmfw::launch_walk_f32x2(desc->rank, A, b, nseg, seg, st)
+                      : cmfw::launch_walk_f64_fast(desc->rank, A, b, nseg, seg, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, ff ? "walk_f32x2 launch" : "walk_f64_fast launch");
     if (owned) {
         hipError_t e2 = hipFreeAsync(owned, st);
         if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
     }
     return release(S, st, rc);
+}
+
+int cmfw_walk_f64_fast(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                       cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                       size_t workspace_bytes, void *stream) {
+    return walk_fast_common(false, desc, in, N, h, out, snaps, workspace, workspace_bytes, stream);
+}
+
+int cmfw_walk_f32x2(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
+                    cmfw_results *out, cmfw_snapshots *snaps, void *workspace,
+                    size_t workspace_bytes, void *stream) {
+    return walk_fast_common(true, desc, in, N, h, out, snaps, workspace, workspace_bytes, stream);
 }
 
 int cmfw_walk_prefix_f64(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K,

@@ -1,0 +1,16 @@
+// walk_f32x2.hip -- the fast engine (walk_fast.hpp) in compensated two-float
+// FP32 (TwoProd through fmaf, TwoSum renormalisation).  FP64 appears only in
+// the exponent bookkeeping's log, the snapshot's max-normalisation, p/q and
+// delta, and in the rare re-decision of a borderline solve denominator in
+// the reference's FP64 operation order (validity parity).
+#include "walk_fast.hpp"
+#include "walk_kernels.hpp"
+
+namespace cmfw {
+
+hipError_t launch_walk_f32x2(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
+                             hipStream_t st) {
+    return launch_walk_fast<ArithFF>(D, A, bounds, nseg, seg_ws, st);
+}
+
+}  // namespace cmfw

@@ -1,0 +1,418 @@
+// walk_fast.hpp -- the CMF walk engineered for the CDNA4 VALU, generic over
+// the arithmetic (ArithF64: double with explicit FMA; ArithFF: compensated
+// two-float FP32).  Same recurrence and validity rules as the reference
+// order (walk_exact.hpp), with:
+//  * depth segments: W_N = S_1 S_2 ... S_K where S_k is the product of the
+//    axis matrices of trajectory steps (b_{k-1}, b_k], started at
+//    start + b_{k-1} * dir, so every (trajectory, segment) pair is an
+//    independent lane; a second kernel multiplies the K segment matrices in
+//    order and takes the snapshots at N-2h, N-h, N (always boundaries);
+//  * power-of-two rescaling instead of the per-step max/divide/log of
+//    normalize_projective: the max is taken every axis step (one max per
+//    entry) and the matrix is rescaled by an exact power of two only when it
+//    leaves the arithmetic's range window; the scale is an integer exponent
+//    and its log is taken once per snapshot, in FP64 (as are the snapshot's
+//    max-normalisation, p/q and delta);
+//  * the theta polynomials rebuilt per axis step with fused multiply-adds,
+//    1/lead and 1/(solve denominator) taken once per step;
+//  * one solve denominator per inverse step (it is row-independent; the
+//    reference recomputes it per row).
+// Validity parity: every failure the reference can raise depends on the
+// position only (lead, denominator, solve denominator).  For integer points
+// lead and denominator are exact in both arithmetics; a solve denominator
+// within 2^20 times its rounding scale of the threshold, or any step with a
+// negative evaluation coordinate (where the theta polynomials can cancel),
+// is re-decided in the reference's FP64 operation order.  Start points that
+// are not integers below 2^23 in magnitude take the reference-order walk.
+#pragma once
+#include "arith.hpp"
+#include "walk_exact.hpp"
+
+namespace cmfw {
+
+template <class Ar>
+struct SegArgsT {
+    using T = typename Ar::T;
+    WalkArgs w;
+    int nseg;
+    int b[kMaxSegments + 1];  // boundaries in trajectory steps, b[0]=0, b[nseg]=N
+    T *segM;                  // [nseg][D][D][n]
+    int32_t *segE;            // [nseg][n] power-of-two exponent of each segment product
+    int32_t *segV;            // [nseg][n] 1 ok, 0 failed, 2 exact path
+    int32_t *segF, *segI;     // [nseg][n] executed fwd / inv axis steps
+};
+
+// Integer points below 2^23: exact in FP32 as well as FP64, and so are the
+// lead sums of the theta polynomials.
+__device__ __forceinline__ bool fast_point(double v) {
+    return v == rint(v) && fabs(v) < 8388608.0;
+}
+
+template <int D, class Ar>
+__device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], const double (&yv)[8],
+                                                      int p, int q, double z,
+                                                      typename Ar::T (&col)[D]) {
+    using T = typename Ar::T;
+    T px[9], py[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        px[k] = Ar::zero();
+        py[k] = Ar::zero();
+    }
+    px[0] = Ar::one();
+    py[0] = Ar::one();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < p) {
+            const T v = Ar::from_d(xv[j]);
+#pragma unroll
+            for (int k = j + 1; k >= 1; --k) px[k] = Ar::fma(px[k], v, px[k - 1]);
+            px[0] = Ar::mul(px[0], v);
+        }
+        if (j < q) {
+            const T v = Ar::from_d(yv[j]);
+#pragma unroll
+            for (int k = j + 1; k >= 1; --k) py[k] = Ar::fma(py[k], v, py[k - 1]);
+            py[0] = Ar::mul(py[0], v);
+        }
+    }
+    const T mz = Ar::from_d(-z);
+    T dc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) dc[k] = Ar::zero();
+#pragma unroll
+    for (int k = 0; k <= 8; ++k)
+        if (k <= p) dc[k] = Ar::mul(mz, px[k]);
+#pragma unroll
+    for (int k = 0; k <= 8; ++k)
+        if (k <= q) dc[k + 1] = Ar::add(dc[k + 1], py[k]);
+    // lead of D(theta) and the reference's vanishing test; for an integer
+    // point the lead is exact in either arithmetic
+    const double lead = Ar::to_d(dc[D]);
+    double lscale = 1.0;
+    if (D == p && p == q + 1) {
+        lscale = fmax(1.0, fabs(1.0 - z));
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < p) lscale += fabs(xv[i]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < q) lscale += fabs(yv[i]);
+    }
+    if (!isfinite(lead) || fabs(lead) <= 64.0 * kEps * lscale) return false;
+    const T nil = Ar::neg(Ar::recip(dc[D]));
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        col[i] = Ar::mul(dc[i], nil);
+        ok = ok && Ar::finite(col[i]);
+    }
+    return ok;
+}
+
+// Keeps max |M| inside [2^-R, 2^R] (R = Ar::kRangeLog2) with exact
+// power-of-two rescales.  Returns false when the max is zero or not finite:
+// the reference's normalize_projective (cuda/cmf_pfq.cuh:109) would then
+// fail -- or, where this arithmetic cancelled exactly (small integers, exact
+// FMA, power-of-two scaling) but the reference's inexact 1/max scaling left a
+// rounding residue, carry on.  The caller hands such a trajectory to the
+// reference-order walk, which decides it exactly as the reference does.
+template <int D, class Ar>
+__device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2, bool force) {
+    using Mg = typename Ar::M;
+    Mg mx = 0;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) mx = fmax(mx, Ar::mag(M[r][c]));
+    if (!(mx > 0) || !isfinite(mx)) return false;
+    const Mg hi = (Mg)ldexp(1.0, Ar::kRangeLog2), lo = (Mg)ldexp(1.0, -Ar::kRangeLog2);
+    if (force || mx > hi || mx < lo) {
+        const int e = ilogb((double)mx) + 1;
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < D; ++c) M[r][c] = Ar::scale2(M[r][c], -e);
+        e2 += e;
+    }
+    return true;
+}
+
+template <int D, class Ar>
+__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const double (&xs)[8],
+                                               const double (&ys)[8], int p, int q, double z,
+                                               int ax, int sign, bool &inverse) {
+    using T = typename Ar::T;
+    double xv[8], yv[8], denom;
+    inverse = axis_eval(xs, ys, p, ax, sign, xv, yv, denom);
+    T col[D];
+    if (!companion_column_fast<D, Ar>(xv, yv, p, q, z, col)) return false;
+    if (denom == 0.0) return false;  // |denom| <= 64 eps for an integer point
+    const T ia = Ar::recip(Ar::from_d(denom));
+    if (!inverse) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            T dot = Ar::mul(M[r][0], col[0]);
+#pragma unroll
+            for (int k = 1; k < D; ++k) dot = Ar::fma(M[r][k], col[k], dot);
+#pragma unroll
+            for (int j = 0; j + 1 < D; ++j) M[r][j] = Ar::fma(M[r][j + 1], ia, M[r][j]);
+            M[r][D - 1] = Ar::fma(dot, ia, M[r][D - 1]);
+        }
+        return true;
+    }
+    T be[D];
+    be[D - 1] = Ar::one();
+#pragma unroll
+    for (int j = D - 2; j >= 0; --j) be[j] = Ar::mul(Ar::neg(be[j + 1]), ia);
+    T db = Ar::mul(be[0], col[0]);
+    double dabs = fabs(Ar::to_d(be[0]) * Ar::to_d(col[0]));
+#pragma unroll
+    for (int k = 1; k < D; ++k) {
+        db = Ar::fma(be[k], col[k], db);
+        dabs += fabs(Ar::to_d(be[k]) * Ar::to_d(col[k]));
+    }
+    T sd = Ar::fma(db, ia, Ar::one());
+    const double aia = fabs(1.0 / denom);
+    // The fast sd decides only when it is far from the reference's threshold
+    // 128 eps max(1, |db/a|): more than 2^20 times the size of its own terms.
+    bool neg = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) neg = neg || (j < p && xv[j] < 0.0) || (j < q && yv[j] < 0.0);
+    if (neg || !(fabs(Ar::to_d(sd)) > 0x1p-20 * (dabs * aia + 1.0))) {
+        // near the threshold: decide exactly as the reference does, in its
+        // FP64 operation order (this file is compiled without implicit
+        // contraction, so the products and sums below round separately)
+        double cref[D];
+        if (!companion_column_ref<D>(xv, yv, p, q, z, cref)) return false;
+        const double iad = 1.0 / denom;
+        double bref[D];
+        bref[D - 1] = 1.0;
+        double dbr = 0.0;
+#pragma unroll
+        for (int j = D - 2; j >= 0; --j) bref[j] = -bref[j + 1] * iad;
+#pragma unroll
+        for (int k = 0; k < D; ++k) dbr = dbr + bref[k] * cref[k];
+        const double sdr = 1.0 + dbr * iad;
+        const double ssr = fmax(1.0, fabs(dbr * iad));
+        if (!isfinite(sdr) || fabs(sdr) <= 128.0 * kEps * ssr) return false;
+        sd = Ar::from_d(sdr);
+#pragma unroll
+        for (int i = 0; i < D; ++i) col[i] = Ar::from_d(cref[i]);
+    }
+    if (!Ar::finite(sd)) return false;
+    const T isd = Ar::recip(sd);
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        T al[D];
+        al[D - 1] = Ar::zero();
+        T da = Ar::zero();
+#pragma unroll
+        for (int j = D - 2; j >= 0; --j) al[j] = Ar::fma(Ar::neg(al[j + 1]), ia, M[r][j]);
+#pragma unroll
+        for (int k = 0; k < D - 1; ++k) da = Ar::fma(al[k], col[k], da);
+        const T tt = Ar::mul(Ar::fma(Ar::neg(da), ia, M[r][D - 1]), isd);
+#pragma unroll
+        for (int j = 0; j < D; ++j) M[r][j] = Ar::fma(be[j], tt, al[j]);
+    }
+    return true;
+}
+
+template <int D, class Ar>
+__global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    const WalkArgs &A = S.w;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (t >= A.n) return;
+    const size_t o = (size_t)s * A.n + t;
+    int32_t dir[16];
+    double xs[8], ys[8];
+    load_traj(A, t, dir, xs, ys);
+    bool clean = true;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) clean = clean && fast_point(xs[j]) && fast_point(ys[j]);
+    if (!clean) {
+        S.segV[o] = 2;
+        S.segF[o] = 0;
+        S.segI[o] = 0;
+        return;
+    }
+    // start of this segment: start + b_s * dir (exact integers)
+    const double b0 = (double)S.b[s];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        xs[j] = fma(b0, (double)(j < A.p ? dir[j] : 0), xs[j]);
+        ys[j] = fma(b0, (double)(j < A.q ? pick(dir, A.p + j) : 0), ys[j]);
+    }
+    T M[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    int e2 = 0, nf = 0, ni = 0;
+    bool ok = true;
+    const int steps = S.b[s + 1] - S.b[s];
+    for (int st = 0; ok && st < steps; ++st) {
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                bool inverse;
+                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse);
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                if (!ok) break;
+                if (!keep_in_range<D, Ar>(M, e2, false)) {
+                    S.segV[o] = 2;  // zero or non-finite product: reference order decides
+                    S.segF[o] = nf;
+                    S.segI[o] = ni;
+                    return;
+                }
+                axis_move(xs, ys, A.p, ax, sign);
+            }
+        }
+    }
+    S.segF[o] = nf;
+    S.segI[o] = ni;
+    S.segV[o] = ok ? 1 : 0;
+    if (!ok) return;
+    if (!keep_in_range<D, Ar>(M, e2, true)) {
+        S.segV[o] = 2;
+        return;
+    }
+    S.segE[o] = e2;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) S.segM[((size_t)((s * D + i) * D + j)) * A.n + t] = M[i][j];
+}
+
+template <int D, class Ar>
+__global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    const WalkArgs &A = S.w;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.n) return;
+    // not an integer point, or a segment product that vanished or overflowed:
+    // the reference-order walk decides this trajectory
+    bool exact = false;
+    for (int s = 0; s < S.nseg; ++s) exact = exact || (S.segV[(size_t)s * A.n + t] == 2);
+    if (exact) {
+        walk_exact_traj<D>(A, t);
+        return;
+    }
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    T W[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) W[i][j] = (i == j) ? Ar::one() : Ar::zero();
+    int e2 = 0, nf = 0, ni = 0;
+    bool ok = !(N <= 0 || h <= 0 || N - 2 * h <= 0);
+    for (int s = 0; ok && s < S.nseg; ++s) {
+        const size_t o = (size_t)s * A.n + t;
+        nf += S.segF[o];
+        ni += S.segI[o];
+        if (S.segV[o] != 1) {
+            ok = false;
+            break;
+        }
+        T G[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * A.n + t];
+        // W <- W * G, one row at a time
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            T row[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                T acc = Ar::mul(W[r][0], G[0][j]);
+#pragma unroll
+                for (int k = 1; k < D; ++k) acc = Ar::fma(W[r][k], G[k][j], acc);
+                row[j] = acc;
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) W[r][j] = row[j];
+        }
+        e2 += S.segE[o];
+        if (!keep_in_range<D, Ar>(W, e2, true)) {
+            walk_exact_traj<D>(A, t);
+            return;
+        }
+        const int bend = S.b[s + 1];
+        for (int k = 0; k < 3; ++k) {
+            const int cp = (k == 0) ? cp1 : (k == 1) ? cp2 : N;
+            if (cp != bend) continue;
+            // the reference's snapshot (FP64): max-normalised, log of the scale
+            double mx = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) mx = fmax(mx, fabs(Ar::to_d(W[i][j])));
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    A.snap[((size_t)((k * D + i) * D + j)) * A.sld + t] = Ar::to_d(W[i][j]) * inv;
+            A.snap_ls[(size_t)k * A.sld + t] = (double)e2 * 0.6931471805599453 + log(mx);
+        }
+    }
+    if (A.n_fwd) A.n_fwd[t] = nf;
+    if (A.n_inv) A.n_inv[t] = ni;
+    finish_from_snapshots<D>(A, t, ok);
+}
+
+template <int D, class Ar>
+hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
+    const int bs = 256;
+    const int gx = (S.w.n + bs - 1) / bs;
+    hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
+    return hipGetLastError();
+}
+
+template <class Ar>
+hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
+                            hipStream_t st) {
+    using T = typename Ar::T;
+    static_assert(sizeof(T) == 8, "segment storage holds 8-byte values");
+    SegArgsT<Ar> S;
+    S.w = A;
+    S.nseg = nseg;
+    for (int i = 0; i <= nseg; ++i) S.b[i] = bounds[i];
+    char *p = (char *)seg_ws;
+    const size_t n = (size_t)A.n;
+    S.segM = (T *)p;
+    p += align256((size_t)nseg * D * D * n * sizeof(T));
+    S.segE = (int32_t *)p;
+    p += align256((size_t)nseg * n * 4);
+    S.segV = (int32_t *)p;
+    p += align256((size_t)nseg * n * 4);
+    S.segF = (int32_t *)p;
+    p += align256((size_t)nseg * n * 4);
+    S.segI = (int32_t *)p;
+    switch (D) {
+        case 2: return launch_fast_d<2, Ar>(S, st);
+        case 3: return launch_fast_d<3, Ar>(S, st);
+        case 4: return launch_fast_d<4, Ar>(S, st);
+        case 5: return launch_fast_d<5, Ar>(S, st);
+        case 6: return launch_fast_d<6, Ar>(S, st);
+        case 7: return launch_fast_d<7, Ar>(S, st);
+        case 8: return launch_fast_d<8, Ar>(S, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cmfw

@@ -28,6 +28,8 @@ hipError_t launch_walk_f64_ref(int D, const WalkArgs &A, hipStream_t st);
 hipError_t launch_walk_f64_fast(int D, const WalkArgs &A, const int *bounds, int nseg,
                                 void *seg_ws, hipStream_t st);
 size_t fast_segment_bytes(int D, int n, int nseg);
+hipError_t launch_walk_f32x2(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
+                             hipStream_t st);
 hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, uint64_t *crt,
                            const int32_t *only, hipStream_t st);
 size_t rns_seg_bytes(int D, int n, int np, int nseg);

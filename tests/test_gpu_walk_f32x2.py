@@ -1,0 +1,120 @@
+"""GPU checks of cmfw_walk_f32x2, the compensated two-float FP32 walk.
+
+Two-float carries ~48 significand bits (rounding ~2^-47 per operation, about
+32x coarser than FP64), so its values are judged against the quad-precision
+walk of the oracle, next to the reference order's own FP64 error.  Validity
+decisions and executed-step counts are identical to the reference's (the
+rare borderline solve denominator is re-decided in its FP64 order)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def errors(g, r, hp):
+    v = (r["valid"] == 1) & (hp["ok"] == 1)
+    exact = hp["ratio"][:, 2]
+    return _rel(g["p"] / g["q"], exact)[v], _rel(r["p"] / r["q"], exact)[v], v
+
+
+@pytest.mark.parametrize("p,q,z", [(3, 2, 0.5), (4, 3, 0.5), (6, 5, 1.0), (6, 5, 0.5),
+                                   (8, 7, 1.0), (8, 7, 0.5)])
+def test_f32x2_validity_and_accuracy(engine, orc, p, q, z):
+    rng = np.random.default_rng(77 * p + int(8 * z))
+    n = 400
+    D = rng.integers(-2, 3, (n, p + q))
+    D[: n // 2] = np.abs(D[: n // 2])
+    D[n // 2: n // 2 + 50] = 0
+    D[n // 2: n // 2 + 50, 0] = 1
+    S = rng.integers(1, 13, (n, p + q)).astype(float)
+    N, h = 200, 25
+    print(f"f32x2 {p}F{q} z={z}", flush=True)
+    g = engine.results_to_numpy(engine.walk_f32x2(p, q, z, D, S, N, h))
+    e = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, N, h))
+    assert (g["valid"] == e["valid"]).all()
+    assert (g["n_fwd"] == e["n_fwd"]).all() and (g["n_inv"] == e["n_inv"]).all()
+    v = e["valid"] == 1
+    assert ((g["n_limits"][v] > 0) == (e["n_limits"][v] > 0)).mean() >= 0.98
+    hp = orc.hp_walk_batch(p, q, z, D, S, N, h)
+    eg, er, vv = errors(g, e, hp)
+    if vv.sum() == 0:
+        return
+    print(f"  {int(vv.sum())} walks, p/q error vs quad (median/max): f32x2 "
+          f"{np.median(eg):.1e}/{eg.max():.1e}, f64 reference {np.median(er):.1e}/{er.max():.1e}",
+          flush=True)
+    # float-float operations are exact to a few ulps of 2^-48, up to ~256x
+    # FP64's half-ulp: a chaotic walk amplifies either by the same factor
+    assert np.median(eg) <= 64.0 * np.median(er) + 1e-12
+    assert eg.max() <= max(256.0 * er.max(), 1e-9)
+
+
+@pytest.mark.parametrize("p,q", [(3, 2), (4, 3)])
+def test_f32x2_well_conditioned_within_1e10(engine, orc, p, q):
+    """Well-conditioned walks (z=1/2, x-only directions, starts 2..9; the
+    reference's own FP64 error there is below 1e-13): every p/q within 1e-10
+    of the quad-precision value, and the fast FP64 engine within 1e-12."""
+    rng = np.random.default_rng(p)
+    n = 200
+    D = np.zeros((n, p + q), int)
+    D[:, :p] = rng.integers(0, 3, (n, p))
+    D[:, 0] = np.maximum(D[:, 0], 1)
+    S = rng.integers(2, 10, (n, p + q)).astype(float)
+    N, h = 300, 50
+    g = engine.results_to_numpy(engine.walk_f32x2(p, q, 0.5, D, S, N, h))
+    f = engine.results_to_numpy(engine.walk_f64_fast(p, q, 0.5, D, S, N, h))
+    hp = orc.hp_walk_batch(p, q, 0.5, D, S, N, h)
+    ok = (g["valid"] == 1) & (hp["ok"] == 1)
+    assert ok.sum() == n
+    eg = _rel(g["p"] / g["q"], hp["ratio"][:, 2])
+    ef = _rel(f["p"] / f["q"], hp["ratio"][:, 2])
+    print(f"  {p}F{q} z=1/2 x-only: p/q error vs quad max f32x2 {eg.max():.1e}, "
+          f"f64 fast {ef.max():.1e}", flush=True)
+    assert eg.max() <= 1e-10
+    assert ef.max() <= 1e-12
+
+
+def test_nilpotent_zeta_walk_hands_over_to_reference_order(engine, orc):
+    """4F3 zeta start, z=1: D(theta) = -(theta+1)^3, so I + C/1 is nilpotent
+    and the exact product vanishes within three steps.  The reference's
+    inexact 1/max scaling leaves a rounding residue and carries on; both fast
+    engines detect the zero matrix and hand the walk to the reference order,
+    so validity and values equal the reference's."""
+    p, q = 4, 3
+    D = np.array([[2, 2, 2, 1, 0, 0, 0], [2, 2, 2, 0, 0, 0, 0], [1, 1, 0, 1, 0, 0, 0]])
+    S = np.tile(np.array([1.0] * p + [2.0] * q), (3, 1))
+    r = orc.walk_f64_batch(p, q, 1.0, D, S, 600, 50)
+    for walk in (engine.walk_f64_fast, engine.walk_f32x2):
+        g = engine.results_to_numpy(walk(p, q, 1.0, D, S, 600, 50))
+        assert (g["valid"] == r["valid"]).all()
+        assert np.array_equal(g["p"], r["p"]) and np.array_equal(g["q"], r["q"])
+
+
+def test_df64_cli_published_counts(engine, published, tmp_path):
+    """cmf_walk_df64 (the two-float engine) reproduces the published df64
+    summary counts of the zeta-limit runs (every trajectory identical and
+    valid) with the reference's flags and JSON keys."""
+    import json
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(__file__), "..", "ramanujandreams-cuda_amd", "bin",
+                       "cmf_walk_df64")
+    for run in published:
+        if run["variant"] != "df64" or run["suite"] != "zeta" or run["mode"] != "limit":
+            continue
+        out = tmp_path / f"df64_{run['cmf']}.json"
+        r = subprocess.run([exe, "--cmf", run["cmf"], "--zeta", "--N", str(run["N"]), "--h",
+                            str(run["h"]), "--n-traj", "2000", "--mode", "limit", "--output",
+                            str(out)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = json.load(open(out))
+        assert d["variant"] == "df64" and d["n_valid"] == 2000
+        assert d["n_with_limits"] == 2000
+        assert set(d) >= {"cmf", "p", "q", "z", "rank", "naxes", "N", "h", "n_traj", "mode",
+                          "block_size", "gpu_time_ms", "throughput_traj_per_s", "mem_used_mb",
+                          "n_valid", "n_with_limits", "avg_delta"}
