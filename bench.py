@@ -1,20 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X CMF walk engine: verified-delta trajectories/s on
-8F7, z=1, N=1000 (zeta start, candidate directions in {0..4}^15, 4,000,000
-candidates from std::mt19937_64(42)).
+8F7, z=1, N=1000, h=50 (zeta start, candidate directions in {0..4}^15,
+4,000,000 candidates from std::mt19937_64(42)), learned proposal vs the
+plain f64 search.
 
-One step = the whole verified-delta pipeline over the candidate batch, inputs
-already resident in HBM:
-  1. screen: the first trajectory step of every candidate (exact: a
-     trajectory that fails there fails the full walk the same way);
-  2. compact the survivors and sort them by direction so the lanes of a wave
-     follow the same axis sequence;
-  3. deep walk: plain f64 (the fast engine: depth segments, FMA, power-of-two
-     rescaling), N=1000, h=50, limits and delta on device;
-  4. verify: the top 1% of candidates by delta walked again in RNS
-     (8 moduli, N=1000) -- all survivors when fewer than 1% survive.
-value = candidates resolved per second.  Kernel times are HIP-event
-measurements on the stream the kernels run on.
+One step = one whole search over the candidate batch, inputs resident in
+HBM.  Both pipelines start with the screen (the position-determined
+validity checks of every candidate's first trajectory step, with the
+reference's decisions: a candidate dropped there fails the walk there or
+earlier) and end with RNS verification (8 moduli, N=1000) of the candidates
+they report.
+  plain f64   : every survivor walked to N=1000 (fast f64 engine), the top
+                1% of the candidates by delta RNS-verified (all survivors
+                when fewer than 1% survive);
+  plain top-V : the same exhaustive walk, only the top V verified (what the
+                proposal saves beyond reporting fewer candidates);
+  proposal    : a 100-step walk of every survivor, the proposal network
+                scores its features, the top M are walked to N=1000 and
+                the top V of those by delta RNS-verified.
+value = candidates resolved per second by the proposal pipeline whose deep
+walk is f64; the same pipeline with two-float FP32 walks is measured beside
+it, with its recall.  Recall@100: fraction of the exhaustive plain walk's
+top-100 by delta (ties at the 100th value included) that the proposal
+verified.  The network is trained before timing (1M walk-generated pairs
+disjoint from the evaluated candidates, GPU kernels of this library).
+Stage and kernel times are HIP events on the stream the kernels run on.
 
 python bench.py [--steps K] [--warmup W] [--gpus 1] [--n-cand N]
 """
@@ -29,7 +39,7 @@ PKG = os.path.join(ROOT, "ramanujandreams-cuda_amd")
 sys.path.insert(0, PKG)
 
 P, Q, Z, N_DEPTH, H = 8, 7, 1.0, 1000, 50
-FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak, AMD spec
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak, AMD spec
 
 
 def canonical_flops(d, p, q):
@@ -42,76 +52,126 @@ def canonical_flops(d, p, q):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-cand", type=int, default=4_000_000)
     ap.add_argument("--top-frac", type=float, default=0.01)
     ap.add_argument("--n-primes", type=int, default=8)
+    ap.add_argument("--deep-m", type=int, default=300, help="proposal: candidates walked deep")
+    ap.add_argument("--verify-v", type=int, default=128, help="proposal: candidates RNS-verified")
+    ap.add_argument("--train-pairs", type=int, default=1_000_000)
+    ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
 
-class Pipeline:
-    def __init__(self, torch, cw, n_cand, top_frac, n_primes):
-        self.torch, self.cw = torch, cw
-        self.desc = cw.make_desc(P, Q, Z)
-        self.n = n_cand
+class Timer:
+    """HIP events recorded on the current stream (where the walks run)."""
+
+    def __init__(self, torch):
+        self.torch, self.ev = torch, {}
+
+    def mark(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.ev.setdefault(name, []).append(e)
+
+    def stages(self, names):
+        n = len(self.ev[names[0]])
+        out = []
+        for i in range(n):
+            out.append({b: self.ev[a][i].elapsed_time(self.ev[b][i])
+                        for a, b in zip(names[:-1], names[1:])})
+        return {k: sum(s[k] for s in out) / n for k in out[0]}
+
+
+class Search:
+    def __init__(self, torch, cw, pr, n_cand, n_primes):
+        self.torch, self.cw, self.pr = torch, cw, pr
         D, S = cw.synth_inputs(42, n_cand, P, Q, 1, 0, 0, 4)
         self.host_D, self.host_S = D, S
-        naxes = P + Q
-        self.dir = torch.from_numpy(D[:, :naxes].T.copy()).cuda()
-        self.start = torch.from_numpy(S[:, :naxes].T.copy()).cuda()
-        self.pre = cw.alloc_results(n_cand)
-        self.n_top = max(1, int(round(top_frac * n_cand)))
+        self.n = n_cand
+        self.dir = torch.from_numpy(D[:, :P + Q].T.copy()).cuda()
+        self.start = torch.from_numpy(S[:, :P + Q].T.copy()).cuda()
+        self.keep = torch.empty(n_cand, dtype=torch.int32, device="cuda")
         self.n_primes = n_primes
-        torch.cuda.synchronize()
 
-    def run(self, timers=None):
-        torch, cw = self.torch, self.cw
-        ev = timers if timers is not None else {}
-
-        def mark(name):
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            ev.setdefault(name, []).append(e)
-
-        mark("t0")
-        cw.walk_prefix_f64(P, Q, Z, self.dir, self.start, 1, out=self.pre)
-        mark("screen")
-        live = torch.nonzero(self.pre["valid"]).squeeze(1)
-        # direction key: axes from the last to the first, 3 bits each, so
-        # neighbours in the sorted order share their first axis steps
-        dl = self.dir[:, live]
-        key = torch.zeros(live.numel(), dtype=torch.int64, device="cuda")
-        for a in range(P + Q - 1, -1, -1):
-            key = key * 8 + dl[a].to(torch.int64)
-        order = torch.argsort(key)
-        live = live[order]
+    def screen_compact(self, T):
+        """Exact screen, then the survivors sorted by direction so that the
+        lanes of a wave follow the same axis sequence."""
+        torch = self.torch
+        live = self.pr.screen(P, Q, Z, self.dir, self.start, keep=self.keep)
+        T.mark("screen")
+        key = self.pr.direction_keys(self.dir[:, live])
+        live = live[torch.argsort(key)]
         dl = self.dir[:, live].contiguous()
         sl = self.start[:, live].contiguous()
-        mark("compact")
+        T.mark("compact")
+        return live, dl, sl
+
+    def plain(self, T, top_frac, top_k=None):
+        """Exhaustive search; verifies the top `top_frac` of the candidates
+        (the reference protocol), or the top `top_k` when given."""
+        cw, torch = self.cw, self.torch
+        T.mark("t0")
+        live, dl, sl = self.screen_compact(T)
         res = cw.walk_f64_fast(P, Q, Z, dl, sl, N_DEPTH, H)
-        mark("deep")
-        delta = res["delta"]
-        ok = (res["valid"] == 1) & torch.isfinite(delta)
-        score = torch.where(ok, delta, torch.full_like(delta, -float("inf")))
-        # candidates outside the survivors rank below every survivor
-        k = min(self.n_top, live.numel())
-        top = torch.topk(score, k).indices if k > 0 else live[:0]
-        dv = dl[:, top].contiguous()
-        sv = sl[:, top].contiguous()
-        mark("select")
-        rns = cw.walk_rns(P, Q, Z, dv, sv, N_DEPTH, H, n_primes=self.n_primes)
-        mark("verify")
+        T.mark("deep")
+        d = res["delta"]
+        score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
+                            torch.full_like(d, -float("inf")))
+        k = top_k if top_k is not None else max(1, int(round(top_frac * self.n)))
+        k = min(k, live.numel())
+        top = torch.topk(score, k).indices
+        T.mark("select")
+        rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
+                          n_primes=self.n_primes)
+        T.mark("verify")
         return {"live": live, "res": res, "top": top, "rns": rns}
 
+    def proposal(self, T, net, engine, M, V):
+        cw, pr, torch = self.cw, self.pr, self.torch
+        T.mark("t0")
+        live, dl, sl = self.screen_compact(T)
+        X, pre = pr.feature_walk(P, Q, Z, dl, sl, engine=engine)
+        T.mark("features")
+        s = net.score(X)
+        m = min(M, live.numel())
+        cand = torch.topk(s, m).indices if m > 0 else live[:0]
+        T.mark("rank")
+        res = pr.WALKS[engine](P, Q, Z, dl[:, cand].contiguous(), sl[:, cand].contiguous(),
+                               N_DEPTH, H)
+        T.mark("deep")
+        d = res["delta"]
+        score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
+                            torch.full_like(d, -float("inf")))
+        v = min(V, m)
+        top = cand[torch.topk(score, v).indices] if v > 0 else cand[:0]
+        T.mark("select")
+        rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
+                          n_primes=self.n_primes)
+        T.mark("verify")
+        return {"live": live, "pre": pre, "cand": cand, "res": res, "top": top, "rns": rns}
 
-def stage_ms(ev, names, i):
-    out = {}
-    for a, b in zip(names[:-1], names[1:]):
-        out[b] = ev[a][i].elapsed_time(ev[b][i])
-    return out
+
+def timed_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn(None)
+    torch.cuda.synchronize()
+    T = Timer(torch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn(T)
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) / steps, T
+
+
+class _NoTimer:
+    def mark(self, name):
+        pass
 
 
 def cpu_baseline(D, S, sample, top_frac, n_primes):
@@ -143,7 +203,8 @@ def cpu_baseline(D, S, sample, top_frac, n_primes):
                           nthreads=threads)
     t2 = time.perf_counter()
     return {"value": n / (t2 - t0), "unit": "traj/s", "cores": threads, "kind": kind,
-            "sample": f"first {n} of the 4M candidates: f64 walk of all ({t1 - t0:.2f}s, "
+            "sample": f"first {n} of the 4M candidates, plain f64 search: f64 walk of all "
+                      f"({t1 - t0:.2f}s, "
                       f"{'reference cmf_pfq.cuh' if kind == 'reference' else 'restatement'}) "
                       f"+ RNS of the top {k} by delta ({t2 - t1:.2f}s, restatement)",
             "survivors": int(alive.size)}
@@ -158,67 +219,129 @@ def main():
     import numpy as np
     import torch
     import cmfwalk as cw
+    from cmfwalk import proposal as pr
 
     cw.load()
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t_setup = time.perf_counter()
-    pipe = Pipeline(torch, cw, args.n_cand, args.top_frac, args.n_primes)
-    print(f"setup {time.perf_counter() - t_setup:.1f}s: {args.n_cand} candidates resident",
-          file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        pipe.run()
-        torch.cuda.synchronize()
-        print(f"warmup {i} done", file=sys.stderr, flush=True)
-    ev = {}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        out = pipe.run(ev)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    names = ["t0", "screen", "compact", "deep", "select", "verify"]
-    stages = [stage_ms(ev, names, i) for i in range(args.steps)]
-    avg = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
-    print(f"stages (ms, avg of {args.steps}): " + json.dumps({k: round(v, 3) for k, v in avg.items()}),
-          file=sys.stderr, flush=True)
+    srch = Search(torch, cw, pr, args.n_cand, args.n_primes)
+    log(f"setup {time.perf_counter() - t_setup:.1f}s: {args.n_cand} candidates resident")
 
-    res = cw.results_to_numpy(out["res"])
-    rns = cw.results_to_numpy(out["rns"])
-    fwd_f, inv_f = canonical_flops(pipe.desc.rank, P, Q)
-    deep_flops = float(res["n_fwd"].astype(np.float64).sum() * fwd_f +
-                       res["n_inv"].astype(np.float64).sum() * inv_f)
-    pre = cw.results_to_numpy(pipe.pre)
-    screen_flops = float(pre["n_fwd"].astype(np.float64).sum() * fwd_f +
-                         pre["n_inv"].astype(np.float64).sum() * inv_f)
-    dominant = max(("screen", "deep", "verify"), key=lambda k: avg[k])
-    ms = elapsed / args.steps * 1e3
-    value = args.n_cand / (elapsed / args.steps)
-    deep_tflops = deep_flops / (avg["deep"] * 1e-3) / 1e12 if avg["deep"] > 0 else 0.0
-    roofline = {"kernel": "walk_f64 (deep)", "bound": "valu", "achieved": deep_tflops,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": deep_tflops / FP64_PEAK_TFLOPS, "traffic": None,
-                "algorithmic_flops_per_launch": deep_flops,
-                "launch_ms": avg["deep"], "dominant_stage": dominant}
+    # ---- proposal network: trained before timing, on pairs disjoint from
+    # the evaluated candidates
+    t_tr = time.perf_counter()
+    live0, dl0, _ = srch.screen_compact(_NoTimer())
+    excl = pr.direction_keys(dl0)
+    Xt, tt, _, _ = pr.make_training_set(P, Q, Z, args.train_pairs, 43, N_DEPTH, H, exclude=excl,
+                                        engine="f64", feature_engine="f64", log=log)
+    net = pr.ProposalNet(seed=0)
+    hist = net.train(Xt, tt, epochs=args.train_epochs, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t_tr
+    log(f"proposal network trained in {train_s:.1f}s: mse {hist[0]:.4f} -> {hist[-1]:.4f}")
+    net_ff = pr.ProposalNet(seed=0)
+    Xf, tf, _, _ = pr.make_training_set(P, Q, Z, args.train_pairs, 43, N_DEPTH, H, exclude=excl,
+                                        engine="f32x2", feature_engine="f32x2")
+    net_ff.train(Xf, tf, epochs=args.train_epochs, batch=4096, lr=2e-3)
+    del Xt, tt, Xf, tf
+
+    # ---- the headline: proposal pipeline, f64 walks
+    names_p = ["t0", "screen", "compact", "features", "rank", "deep", "select", "verify"]
+    out_p, sec_p, T_p = timed_steps(
+        torch, lambda T: srch.proposal(T or _NoTimer(), net, "f64", args.deep_m, args.verify_v),
+        args.steps, args.warmup)
+    st_p = T_p.stages(names_p)
+    log("proposal f64 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_p.items()}))
+    # ---- the same pipeline with two-float walks
+    out_f, sec_f, T_f = timed_steps(
+        torch, lambda T: srch.proposal(T or _NoTimer(), net_ff, "f32x2", args.deep_m,
+                                       args.verify_v), args.steps, args.warmup)
+    st_f = T_f.stages(names_p)
+    log("proposal f32x2 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_f.items()}))
+    # ---- plain f64 search (exhaustive deep walk; the judge of recall)
+    names_b = ["t0", "screen", "compact", "deep", "select", "verify"]
+    out_b, sec_b, T_b = timed_steps(torch, lambda T: srch.plain(T or _NoTimer(), args.top_frac),
+                                    args.steps, args.warmup)
+    st_b = T_b.stages(names_b)
+    log("plain f64 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_b.items()}))
+    # ---- plain f64 search reporting only its top V (isolates the network's
+    # own gain: the deep walk of the M proposed instead of every survivor)
+    out_v, sec_v, T_v = timed_steps(
+        torch, lambda T: srch.plain(T or _NoTimer(), args.top_frac, top_k=args.verify_v),
+        args.steps, args.warmup)
+    st_v = T_v.stages(names_b)
+    log("plain f64, top-V verified, stages (ms): " +
+        json.dumps({k: round(v, 3) for k, v in st_v.items()}))
+
+    exhaustive = cw.results_to_numpy(out_b["res"])["delta"]
+    assert torch.equal(out_b["live"], out_p["live"])
+    rec_p = pr.recall_at(100, exhaustive, out_p["top"].cpu().numpy())
+    rec_f = pr.recall_at(100, exhaustive, out_f["top"].cpu().numpy())
+    rns_p = cw.results_to_numpy(out_p["rns"])
+
+    # roofline of the headline pipeline's dominant stage
+    fwd_f, inv_f = canonical_flops(7, P, Q)
+    steps = torch.empty(args.n_cand, dtype=torch.int32, device="cuda")
+    cw.screen(P, Q, Z, srch.dir, srch.start, 1, steps=steps)
+    # the screen evaluates the companion column (C + d divisions) and the
+    # step denominator per axis step, no matrix work
+    c_col = canonical_flops(7, P, Q)[0] - 5 * 7 * 7 - 7 - 2 + 7 + 2
+    screen_flops = float(steps.to(torch.float64).sum().item()) * c_col
+    deep = cw.results_to_numpy(out_p["res"])
+    deep_flops = float(deep["n_fwd"].astype(np.float64).sum() * fwd_f +
+                       deep["n_inv"].astype(np.float64).sum() * inv_f)
+    feat = cw.results_to_numpy(out_p["pre"])
+    feat_flops = float(feat["n_fwd"].astype(np.float64).sum() * fwd_f +
+                       feat["n_inv"].astype(np.float64).sum() * inv_f)
+    flops = {"screen": screen_flops, "deep": deep_flops, "features": feat_flops}
+    kern = {"screen": "k_screen<7>", "deep": "k_walk_seg<7,f64> + combine",
+            "features": "k_walk_seg<7,f64> + combine (100 steps)"}
+    dom = max(("screen", "features", "deep", "verify"), key=lambda k: st_p[k])
+    fl = flops.get(dom)
+    ach = fl / (st_p[dom] * 1e-3) / 1e12 if fl else None
+    roofline = {"kernel": kern.get(dom, "k_rns_segment<7> + k_rns_combine<7>"),
+                "stage": dom, "bound": "valu",
+                "achieved": ach, "peak": FP64_PEAK_TFLOPS if fl else None,
+                "unit": "TFLOP/s" if fl else None,
+                "frac": ach / FP64_PEAK_TFLOPS if fl else None, "traffic": None,
+                "algorithmic_flops_per_launch": fl, "launch_ms": st_p[dom]}
+
+    value = args.n_cand / sec_p
     line = {
         "metric": "verified-delta traj/s, 8F7 z=1 N=1000: proposal+2xFP32 vs plain f64",
         "value": value, "unit": "traj/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": sec_p * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "8F7 z=1 N=1000 h=50, zeta start, dirs {0..4}^15, "
-                               f"{args.n_cand} candidates, top {args.top_frac:.0%} RNS-verified "
+        "config": {"workload": f"8F7 z=1 N=1000 h=50, zeta start, dirs {{0..4}}^15, "
+                               f"{args.n_cand} candidates (mt19937_64 seed 42)",
+                   "pipeline": f"proposal: screen, 100-step f64 feature walk, MLP rank, top "
+                               f"{args.deep_m} walked f64, top {args.verify_v} RNS-verified "
                                f"({args.n_primes} moduli)",
-                   "pipeline": "plain-f64 (screen, deep f64 walk, RNS verify)",
                    "parallelism": "single-gpu"},
-        "stages_ms": {k: round(v, 3) for k, v in avg.items()},
-        "survivors": int(out["live"].numel()),
-        "finite_delta": int(((res["valid"] == 1) & np.isfinite(res["delta"])).sum()),
-        "verified": int(out["top"].numel()),
-        "rns_valid": int(rns["valid"].sum()),
-        "screen_flops": screen_flops,
+        "speedup_vs_plain_f64": sec_b / sec_p,
+        "recall_at_100": rec_p,
+        "stages_ms": {k: round(v, 3) for k, v in st_p.items()},
+        "plain_f64": {"value": args.n_cand / sec_b, "ms_per_step": sec_b * 1e3,
+                      "stages_ms": {k: round(v, 3) for k, v in st_b.items()},
+                      "verified": int(out_b["top"].numel())},
+        "plain_f64_top_v": {"value": args.n_cand / sec_v, "ms_per_step": sec_v * 1e3,
+                            "stages_ms": {k: round(v, 3) for k, v in st_v.items()},
+                            "verified": int(out_v["top"].numel()),
+                            "proposal_speedup": sec_v / sec_p},
+        "proposal_f32x2": {"value": args.n_cand / sec_f, "ms_per_step": sec_f * 1e3,
+                           "speedup_vs_plain_f64": sec_b / sec_f, "recall_at_100": rec_f,
+                           "stages_ms": {k: round(v, 3) for k, v in st_f.items()}},
+        "survivors": int(out_p["live"].numel()),
+        "verified": int(out_p["top"].numel()),
+        "rns_valid": int(rns_p["valid"].sum()),
+        "training": {"pairs": args.train_pairs, "epochs": args.train_epochs,
+                     "seconds_incl_data": round(train_s, 2), "mse_first_epoch": hist[0],
+                     "mse_last_epoch": hist[-1]},
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        print("cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(pipe.host_D, pipe.host_S, args.cpu_sample,
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(srch.host_D, srch.host_S, args.cpu_sample,
                                             args.top_frac, args.n_primes)
     else:
         line["cpu_baseline"] = None

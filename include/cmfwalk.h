@@ -172,14 +172,72 @@ int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t st
                       int32_t start_hi, int32_t dir_lo, int32_t dir_hi, int32_t *dir_out,
                       double *start_out);
 
+/* Candidate screen: keep[t] = 0 when trajectory t certainly fails within
+ * its first K trajectory steps, decided by the position-determined checks of
+ * the walk (lead coefficient of the companion column, step denominator,
+ * solve denominator of inverse steps) with the reference's operations in
+ * its order; keep[t] = 1 otherwise (such a trajectory may still fail on a
+ * matrix-state check, which a walk decides).  Start points off the integer
+ * grid are always kept.  Same failure checks as forward_walk's
+ * (cuda/cmf_pfq.cuh:348-371) without the matrix products; used to screen
+ * large candidate sets before walking them.  keep: device int32 [n];
+ * steps (nullable): device int32 [n], axis steps evaluated. */
+int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t *keep,
+                int32_t *steps, void *stream);
+
 /* Process-wide tuning knobs (read when a walk is launched; not for
  * concurrent use with running calls).  Keys:
  *   "rns_segments"      1 (default) walk RNS depth segments in parallel,
  *                       0 walk every trajectory sequentially;
- *   "target_lanes"      lanes a fast f64 walk aims to fill (262144);
- *   "min_segment_steps" shortest fast f64 depth segment (25).
+ *   "target_lanes"      lanes an RNS walk aims to fill (262144):
+ *                       trajectories x modulus lanes x segments;
+ *   "fast_segments"     depth segments of a fast f64 / two-float walk (32);
+ *                       the plan depends on N and h only, so a trajectory's
+ *                       result does not depend on the batch it is walked in;
+ *   "min_segment_steps" shortest fast f64 / two-float depth segment (8);
+ *   "rns_min_segment_steps" shortest RNS segment between two boundaries
+ *                       (4); every CRT point is a boundary.
  * No counterpart in the reference. */
 int cmfw_set_tuning(const char *key, int64_t value);
+
+/* Proposal network ------------------------------------------------------
+ * A 64 -> 256 -> 256 -> 256 -> 1 ReLU MLP that scores candidate
+ * trajectories from their start, direction, z and the outcome of a short
+ * prefix walk, so only the best-scored are walked deep and RNS-verified.
+ * Hand-written BF16 MFMA GEMMs with FP32 accumulation; FP32 master weights
+ * and Adam.  None of this has a counterpart in the reference.
+ * params: fp32 [cmfw_mlp_param_count()] laid out W1[64][256], W2[256][256],
+ * W3[256][256], w4[256], b1[256], b2[256], b3[256], b4;
+ * params_bf16: bf16 bits [cmfw_mlp_param_bf16_count()] = W1..W3.
+ * X: bf16 bits [batch][64].  All pointers are device pointers. */
+int64_t cmfw_mlp_param_count(void);
+int64_t cmfw_mlp_param_bf16_count(void);
+size_t cmfw_mlp_workspace_bytes(int32_t batch);
+/* y[batch] = network(X) */
+int cmfw_mlp_forward(const float *params, const uint16_t *params_bf16, const uint16_t *X,
+                     int32_t batch, float *y, void *workspace, size_t workspace_bytes,
+                     void *stream);
+/* gradient of mean((y - target)^2) into grad[param_count]; sq_rows
+ * (nullable) receives (y - target)^2 per row */
+int cmfw_mlp_grad(const float *params, const uint16_t *params_bf16, const uint16_t *X,
+                  const float *target, int32_t batch, float *grad, float *sq_rows,
+                  void *workspace, size_t workspace_bytes, void *stream);
+/* one Adam step (beta 0.9/0.999, eps 1e-8; step counts from 1) on params,
+ * refreshing params_bf16 */
+int cmfw_mlp_adam(float *params, uint16_t *params_bf16, const float *grad, float *m, float *v,
+                  int32_t step, float lr, void *stream);
+int cmfw_mlp_sync_bf16(const float *params, uint16_t *params_bf16, void *stream);
+/* features [n][64] of candidates from their inputs and a K-step walk's
+ * results (valid, n_fwd, n_inv, log_scale, delta, precision, limit,
+ * n_limits, p, q must be set).  Direction/start pairs are sorted within the
+ * x group and the y group (the pFq walk is symmetric under those
+ * permutations), so permuted candidates get identical rows. */
+int cmfw_mlp_features(const cmfw_desc *desc, const cmfw_inputs *in, const cmfw_results *prefix,
+                      int32_t K, uint16_t *X, void *stream);
+/* the raw BF16 MFMA GEMM, C[M][N] fp32 = op(A) op(B); mode bit 0: B stored
+ * [N][ldb] (transposed), bit 1: A stored [K][lda] (transposed).  For tests. */
+int cmfw_mlp_gemm(int32_t mode, int32_t M, int32_t N, int32_t K, const uint16_t *A, int32_t lda,
+                  const uint16_t *B, int32_t ldb, float *C, int32_t ldc, void *stream);
 
 /* Modulus k of the reference table (cuda/cmf_walk_rns.cu:26-43), 0 if out
  * of range. */

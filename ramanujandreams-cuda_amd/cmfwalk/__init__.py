@@ -68,6 +68,22 @@ SIGNATURES = {
                                        C.POINTER(Results), C.c_void_p]),
     "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "cmfw_screen": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_void_p,
+                              C.c_void_p, C.c_void_p]),
+    "cmfw_mlp_param_count": (C.c_int64, []),
+    "cmfw_mlp_param_bf16_count": (C.c_int64, []),
+    "cmfw_mlp_workspace_bytes": (C.c_size_t, [C.c_int32]),
+    "cmfw_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                   C.c_void_p, C.c_size_t, C.c_void_p]),
+    "cmfw_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "cmfw_mlp_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int32, C.c_float, C.c_void_p]),
+    "cmfw_mlp_sync_bf16": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cmfw_mlp_features": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.POINTER(Results),
+                                    C.c_int32, C.c_void_p, C.c_void_p]),
+    "cmfw_mlp_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                                C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     "cmfw_walk_rns": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                 C.c_int32, C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p,
                                 C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
@@ -254,6 +270,22 @@ def walk_prefix_f64(p, q, z, dirs, starts, K, out=None, stream=None):
     return out
 
 
+def screen(p, q, z, dirs, starts, K=1, keep=None, steps=None, stream=None):
+    """int32 [n] keep flags: 0 where the first K trajectory steps certainly
+    fail on a position-determined check (exact reference decisions).  With
+    a `steps` tensor, the axis steps evaluated per candidate are written."""
+    import torch
+
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    keep = keep if keep is not None else torch.empty(n, dtype=torch.int32, device=dt.device)
+    _check(L.cmfw_screen(C.byref(d), C.byref(_inputs_struct(dt, st)), K, keep.data_ptr(),
+                         None if steps is None else steps.data_ptr(), _stream_ptr(stream)))
+    return keep
+
+
 def synth_inputs(seed, n, p, q, start_lo, start_hi, dir_lo, dir_hi):
     """Host arrays ([n, 16] int32 dirs, [n, 16] float64 starts) from the
     library's mt19937_64 generator (start_lo > start_hi: zeta start)."""
@@ -292,6 +324,96 @@ def walk_rns(p, q, z, dirs, starts, N, h, n_primes=8, out=None, snapshots=None,
     if want_resid or want_crt:
         return out, resid, crt
     return out
+
+
+# ---------------------------------------------------------------- network --
+# Proposal-network buffers are torch tensors on the GPU: parameters float32
+# [mlp_param_count()], their bf16 copy torch.bfloat16 [mlp_param_bf16_count()],
+# features torch.bfloat16 [B, 64].
+
+def mlp_param_count():
+    return int(load_library().cmfw_mlp_param_count())
+
+
+def mlp_param_bf16_count():
+    return int(load_library().cmfw_mlp_param_bf16_count())
+
+
+def mlp_workspace(batch, device="cuda"):
+    import torch
+
+    nb = int(load_library().cmfw_mlp_workspace_bytes(batch))
+    return torch.empty(nb, dtype=torch.uint8, device=device)
+
+
+def _ws_args(ws):
+    return (None, 0) if ws is None else (ws.data_ptr(), ws.numel())
+
+
+def mlp_forward(params, params_bf16, X, y=None, workspace=None, stream=None):
+    import torch
+
+    L = load()
+    B = X.shape[0]
+    y = y if y is not None else torch.empty(B, dtype=torch.float32, device=X.device)
+    ws, wsb = _ws_args(workspace)
+    _check(L.cmfw_mlp_forward(params.data_ptr(), params_bf16.data_ptr(), X.data_ptr(), B,
+                              y.data_ptr(), ws, wsb, _stream_ptr(stream)))
+    return y
+
+
+def mlp_grad(params, params_bf16, X, target, grad=None, sq_rows=None, workspace=None,
+             stream=None):
+    """Gradient of mean((network(X) - target)^2); returns (grad, sq_rows)."""
+    import torch
+
+    L = load()
+    B = X.shape[0]
+    grad = grad if grad is not None else torch.empty_like(params)
+    sq_rows = sq_rows if sq_rows is not None else torch.empty(B, dtype=torch.float32,
+                                                              device=X.device)
+    ws, wsb = _ws_args(workspace)
+    _check(L.cmfw_mlp_grad(params.data_ptr(), params_bf16.data_ptr(), X.data_ptr(),
+                           target.data_ptr(), B, grad.data_ptr(), sq_rows.data_ptr(), ws, wsb,
+                           _stream_ptr(stream)))
+    return grad, sq_rows
+
+
+def mlp_adam(params, params_bf16, grad, m, v, step, lr, stream=None):
+    _check(load().cmfw_mlp_adam(params.data_ptr(), params_bf16.data_ptr(), grad.data_ptr(),
+                                m.data_ptr(), v.data_ptr(), int(step), float(lr),
+                                _stream_ptr(stream)))
+
+
+def mlp_sync_bf16(params, params_bf16, stream=None):
+    _check(load().cmfw_mlp_sync_bf16(params.data_ptr(), params_bf16.data_ptr(),
+                                     _stream_ptr(stream)))
+
+
+def mlp_features(p, q, z, dirs, starts, prefix, K, X=None, stream=None):
+    """Features [n, 64] (bf16) of candidates from a K-step walk's results."""
+    import torch
+
+    L = load()
+    d = make_desc(p, q, z)
+    dt, st = to_device_inputs(dirs, starts)
+    n = dt.shape[1]
+    X = X if X is not None else torch.empty((n, 64), dtype=torch.bfloat16, device=dt.device)
+    _check(L.cmfw_mlp_features(C.byref(d), C.byref(_inputs_struct(dt, st)),
+                               C.byref(_results_struct(prefix)), K, X.data_ptr(),
+                               _stream_ptr(stream)))
+    return X
+
+
+def mlp_gemm(mode, A, B, M, N, K, lda, ldb, stream=None):
+    """Raw BF16 MFMA GEMM (tests): C[M, N] float32; mode bit 0 = B stored
+    transposed ([N][ldb]), bit 1 = A stored transposed ([K][lda])."""
+    import torch
+
+    Cm = torch.full((M, N), float("nan"), dtype=torch.float32, device=A.device)
+    _check(load().cmfw_mlp_gemm(mode, M, N, K, A.data_ptr(), lda, B.data_ptr(), ldb,
+                                Cm.data_ptr(), N, _stream_ptr(stream)))
+    return Cm
 
 
 def results_to_numpy(out):

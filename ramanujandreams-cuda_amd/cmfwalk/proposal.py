@@ -1,0 +1,236 @@
+"""Learned proposal for the CMF walk: a small MLP ranks candidate
+trajectories so that only the best-ranked are walked deep and RNS-verified.
+
+The reference walks every candidate to full depth (benchmarks/run_benchmark.py
+drives cmf_walk_f64 over whole batches); this module is the engine's own
+addition.  The network (64 -> 256 -> 256 -> 256 -> 1, ReLU) is trained and
+evaluated on the GPU by the library's BF16 MFMA kernels (cmfw_mlp_*); its
+inputs are the candidate's start point, direction and z plus the outcome of
+a short compensated two-float walk (FEATURE_STEPS trajectory steps): delta,
+limit, precision, log scale, validity and executed-step fraction.  Training
+pairs are generated by the walk itself: candidates drawn on the GPU, the
+exact one-step screen, the feature walk, and the full-depth walk
+whose delta is the regression target.  The RNS arithmetic that verifies the
+final set is never approximated: the network only decides which candidates
+are walked deep, and the exhaustive walk stays available as fallback and as
+the judge of the ranking's recall (recall_at).
+"""
+import math
+
+import numpy as np
+
+from . import (load, mlp_adam, mlp_features, mlp_forward, mlp_grad, mlp_param_bf16_count,
+               mlp_param_count, mlp_sync_bf16, mlp_workspace, walk_f32x2, walk_f64_fast)
+from . import screen as cw_screen
+
+FEATURE_STEPS = 100   # trajectory steps of the feature walk
+FEATURE_H = 25        # its snapshot spacing (delta from steps 50, 75, 100)
+N_IN, N_HID = 64, 256
+TARGET_LO, TARGET_HI = -1.0, 4.0   # regression target: delta clipped to this range
+
+
+def init_params(seed):
+    """He-normal matrices (fan-in), zero biases, as one flat float32 vector
+    in the library's layout W1, W2, W3, w4, b1, b2, b3, b4."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal((N_IN, N_HID)) * math.sqrt(2.0 / N_IN),
+             rng.standard_normal((N_HID, N_HID)) * math.sqrt(2.0 / N_HID),
+             rng.standard_normal((N_HID, N_HID)) * math.sqrt(2.0 / N_HID),
+             rng.standard_normal(N_HID) * math.sqrt(1.0 / N_HID),
+             np.zeros(3 * N_HID + 1)]
+    flat = np.concatenate([a.ravel() for a in parts]).astype(np.float32)
+    assert flat.size == mlp_param_count()
+    return flat
+
+
+def delta_target(results):
+    """Regression target from walk results: delta clipped to
+    [TARGET_LO, TARGET_HI]; invalid walks and non-finite deltas map to
+    TARGET_LO (nothing to verify there)."""
+    import torch
+
+    d = results["delta"]
+    ok = (results["valid"] == 1) & torch.isfinite(d)
+    t = torch.where(ok, d, torch.full_like(d, TARGET_LO))
+    return t.clamp(TARGET_LO, TARGET_HI).to(torch.float32)
+
+
+def sample_candidates(n, p, q, dir_hi_x, dir_hi_y, generator, device="cuda"):
+    """n candidate directions on the GPU, [p+q, n] int32, x-axis entries
+    uniform in {0..dir_hi_x} and y-axis entries in {0..dir_hi_y}; the
+    all-zero direction becomes e_1 (as the library's own generator does)."""
+    import torch
+
+    d = torch.empty((p + q, n), dtype=torch.int32, device=device)
+    d[:p] = torch.randint(0, dir_hi_x + 1, (p, n), generator=generator, device=device,
+                          dtype=torch.int32)
+    if q:
+        d[p:] = torch.randint(0, dir_hi_y + 1, (q, n), generator=generator, device=device,
+                              dtype=torch.int32)
+    zero = (d == 0).all(dim=0)
+    d[0] = torch.where(zero, torch.ones_like(d[0]), d[0])
+    return d
+
+
+def zeta_starts(p, q, n, device="cuda"):
+    """The zeta start point x = 1, y = 2 for n trajectories, [p+q, n]."""
+    import torch
+
+    s = torch.empty((p + q, n), dtype=torch.float64, device=device)
+    s[:p] = 1.0
+    s[p:] = 2.0
+    return s
+
+
+def direction_keys(dirs, base=8):
+    """One int64 key per direction column (entries must lie in 0..base-1)."""
+    import torch
+
+    key = torch.zeros(dirs.shape[1], dtype=torch.int64, device=dirs.device)
+    for a in range(dirs.shape[0] - 1, -1, -1):
+        key = key * base + dirs[a].to(torch.int64)
+    return key
+
+
+def screen(p, q, z, dirs, starts, keep=None, steps=None):
+    """Indices of the candidates kept by the position-determined checks of
+    the first trajectory step (cmfw_screen: a dropped candidate fails the
+    walk there or earlier, with the reference's own decision)."""
+    import torch
+
+    keep = cw_screen(p, q, z, dirs, starts, 1, keep=keep, steps=steps)
+    return torch.nonzero(keep).squeeze(1)
+
+
+WALKS = {"f32x2": walk_f32x2, "f64": walk_f64_fast}
+
+
+def feature_walk(p, q, z, dirs, starts, out=None, X=None, engine="f32x2"):
+    """The short walk (two-float or fast f64; same validity decisions) and
+    the feature rows built from it."""
+    pre = WALKS[engine](p, q, z, dirs, starts, FEATURE_STEPS, FEATURE_H, out=out)
+    return mlp_features(p, q, z, dirs, starts, pre, FEATURE_STEPS, X=X), pre
+
+
+class ProposalNet:
+    """FP32 master weights + bf16 copy + Adam state, all on the GPU."""
+
+    def __init__(self, seed=0, device="cuda"):
+        import torch
+
+        load()
+        self.device = device
+        self.params = torch.from_numpy(init_params(seed)).to(device)
+        self.params_bf16 = torch.empty(mlp_param_bf16_count(), dtype=torch.bfloat16,
+                                       device=device)
+        mlp_sync_bf16(self.params, self.params_bf16)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.step = 0
+        self._ws = {}
+
+    def _workspace(self, batch):
+        if batch not in self._ws:
+            self._ws[batch] = mlp_workspace(batch, self.device)
+        return self._ws[batch]
+
+    def train(self, X, target, epochs=4, batch=4096, lr=2e-3, lr_end=1e-4, seed=0,
+              log=None):
+        """Mini-batch Adam on mean squared error, cosine learning-rate decay
+        from lr to lr_end; rows are reshuffled every epoch on the GPU
+        (deterministic for a seed).  Returns the mean loss of each epoch."""
+        import torch
+
+        n = X.shape[0]
+        nb = n // batch
+        if nb == 0:
+            raise ValueError(f"{n} training rows < one batch of {batch}")
+        g = torch.Generator(device=self.device)
+        g.manual_seed(seed)
+        ws = self._workspace(batch)
+        grad = torch.empty_like(self.params)
+        sq = torch.empty(batch, dtype=torch.float32, device=self.device)
+        total = epochs * nb
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=g, device=self.device)
+            acc = torch.zeros((), dtype=torch.float64, device=self.device)
+            for b in range(nb):
+                idx = perm[b * batch:(b + 1) * batch]
+                xb = X[idx].contiguous()
+                tb = target[idx].contiguous()
+                mlp_grad(self.params, self.params_bf16, xb, tb, grad=grad, sq_rows=sq,
+                         workspace=ws)
+                acc += sq.sum(dtype=torch.float64)
+                it = ep * nb + b
+                lr_t = lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * it / total))
+                self.step += 1
+                mlp_adam(self.params, self.params_bf16, grad, self.m, self.v, self.step, lr_t)
+            hist.append(acc.item() / (nb * batch))
+            if log:
+                log(f"epoch {ep}: mse {hist[-1]:.5f}")
+        return hist
+
+    def score(self, X, out=None, chunk=1 << 18):
+        import torch
+
+        n = X.shape[0]
+        out = out if out is not None else torch.empty(n, dtype=torch.float32,
+                                                      device=self.device)
+        for s in range(0, n, chunk):
+            e = min(n, s + chunk)
+            mlp_forward(self.params, self.params_bf16, X[s:e], y=out[s:e],
+                        workspace=self._workspace(min(chunk, n)))
+        return out
+
+
+def make_training_set(p, q, z, n_pairs, seed, N, h, dir_hi_x=4, dir_hi_y=2, exclude=None,
+                      chunk=1 << 22, engine="f64", feature_engine="f32x2", log=None):
+    """Walk-generated (features, target) pairs on the GPU: candidates drawn
+    from `seed`, screened, and the survivors walked (feature walk and
+    full-depth walk in `engine`).  Directions whose key is in `exclude` (an
+    int64 tensor of direction_keys) are dropped, so the pairs can be kept
+    disjoint from an evaluation set.  Returns (X bf16 [n, 64], target f32 [n],
+    dirs int32 [p+q, n], deep results)."""
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    keep = torch.empty(chunk, dtype=torch.int32, device="cuda")
+    kept = []
+    have = drawn = 0
+    while have < n_pairs:
+        d = sample_candidates(chunk, p, q, dir_hi_x, dir_hi_y, g)
+        s = zeta_starts(p, q, chunk)
+        live = screen(p, q, z, d, s, keep=keep)
+        drawn += chunk
+        dl = d[:, live]
+        if exclude is not None and exclude.numel():
+            dl = dl[:, ~torch.isin(direction_keys(dl), exclude)]
+        kept.append(dl)
+        have += dl.shape[1]
+        if drawn > 4000 * max(n_pairs, 1) + 64 * chunk:
+            raise RuntimeError(f"only {have} survivors in {drawn} candidates")
+    dirs = torch.cat(kept, dim=1)[:, :n_pairs].contiguous()
+    n = dirs.shape[1]
+    starts = zeta_starts(p, q, n)
+    X, _ = feature_walk(p, q, z, dirs, starts, engine=feature_engine)
+    deep = WALKS[engine](p, q, z, dirs, starts, N, h)
+    if log:
+        log(f"training set: {n} survivors of {drawn} candidates")
+    return X, delta_target(deep), dirs, deep
+
+
+def recall_at(k, exhaustive_delta, chosen):
+    """Recall of the top-k set: with tau the k-th largest finite delta of
+    the exhaustive walk, the fraction of candidates with delta >= tau (ties
+    included) that appear in `chosen` (indices into the same candidate
+    list)."""
+    d = np.asarray(exhaustive_delta, dtype=np.float64)
+    fin = np.isfinite(d)
+    if fin.sum() == 0:
+        return 1.0
+    k = min(k, int(fin.sum()))
+    tau = np.sort(d[fin])[::-1][k - 1]
+    top = np.nonzero(fin & (d >= tau))[0]
+    return float(np.isin(top, np.asarray(chosen)).mean())

@@ -60,20 +60,23 @@ size_t snap_bytes(int D, int n) {
 // Lanes that keep every SIMD of an MI355X busy: 256 CUs x 4 SIMDs x 64 lanes
 // x 4 waves.  Fewer trajectories than that are split along their depth.
 int64_t g_target_lanes = 262144;
-int g_min_segment_steps = 25;
+int g_min_segment_steps = 8;
+int g_fast_segments = 32;
+int g_rns_min_segment_steps = 4;
 bool g_rns_segments = true;
 
 // Depth segments of a fast walk: multiples of a segment length L plus the
-// snapshot depths N-2h, N-h, N; b[0] = 0, b[nseg] = N.
-int plan_segments(int n, int N, int h, int *b) {
+// snapshot depths N-2h, N-h, N; b[0] = 0, b[nseg] = N.  L depends on N
+// alone (about g_fast_segments segments, at least g_min_segment_steps
+// steps each), never on the batch: a trajectory's fast-walk result is the
+// same whichever batch it is walked in.
+int plan_segments(int N, int h, int *b) {
     if (N <= 0 || h <= 0 || N - 2 * h <= 0) {
         b[0] = 0;
         b[1] = 0;
         return 1;  // the walk is invalid; one empty segment
     }
-    int64_t want = (g_target_lanes + n - 1) / (n > 0 ? n : 1);
-    if (want < 1) want = 1;
-    int L = (int)((N + want - 1) / want);
+    int L = (N + g_fast_segments - 1) / g_fast_segments;
     if (L < g_min_segment_steps) L = g_min_segment_steps;
     for (;;) {
         int cnt = 0;
@@ -208,11 +211,13 @@ size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, in
                             int32_t n_primes) {
     if (!desc || n_traj < 0) return 0;
     int b[cmfw::kMaxSegments + 1];
-    const int nseg = plan_segments(n_traj, N, h, b);
+    const int nseg = plan_segments(N, h, b);
     size_t need = cmfw::fast_segment_bytes(desc->rank, n_traj, nseg);
     if (n_primes >= 1 && n_primes <= CMFW_MAX_PRIMES) {
         int rb[cmfw::kMaxRnsSegments + 1];
-        const int rseg = cmfw::rns_plan(N, h, rb, cmfw::kMaxRnsSegments);
+        uint8_t rc[cmfw::kMaxRnsSegments + 1];
+        const int rseg = cmfw::rns_plan(N, h, n_traj, n_primes, g_target_lanes,
+                                        g_rns_min_segment_steps, rb, rc, cmfw::kMaxRnsSegments);
         if (rseg >= 0) {
             const size_t r = cmfw::rns_seg_bytes(desc->rank, n_traj, n_primes, rseg);
             need = need > r ? need : r;
@@ -252,7 +257,7 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
     hipStream_t st = (hipStream_t)stream;
     const int n = in->n_traj;
     int b[cmfw::kMaxSegments + 1];
-    const int nseg = plan_segments(n, N, h, b);
+    const int nseg = plan_segments(N, h, b);
     const size_t seg_need = cmfw::fast_segment_bytes(desc->rank, n, nseg);
     SnapStore S;
     void *seg = nullptr, *owned = nullptr;
@@ -356,7 +361,11 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     hipStream_t st = (hipStream_t)stream;
     const int n = in->n_traj;
     int rb[cmfw::kMaxRnsSegments + 1];
-    const int rseg = g_rns_segments ? cmfw::rns_plan(N, h, rb, cmfw::kMaxRnsSegments) : -1;
+    uint8_t rcp[cmfw::kMaxRnsSegments + 1];
+    const int rseg = g_rns_segments
+                         ? cmfw::rns_plan(N, h, n, n_primes, g_target_lanes,
+                                          g_rns_min_segment_steps, rb, rcp, cmfw::kMaxRnsSegments)
+                         : -1;
     cmfw::WalkArgs A;
     SnapStore S;
     if (rseg < 0) {  // depth beyond the segment table: the sequential kernel alone
@@ -383,7 +392,7 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     }
     fill_args(A, desc, in, N, h, out, S);
     int32_t *fallback = nullptr;
-    hipError_t e = cmfw::launch_walk_rns_seg(desc->rank, A, n_primes, rb, rseg, seg, resid_out,
+    hipError_t e = cmfw::launch_walk_rns_seg(desc->rank, A, n_primes, rb, rcp, rseg, seg, resid_out,
                                              crt_out, &fallback, st);
     if (e == hipSuccess && rseg > 0)  // start points off the integer grid
         e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, fallback, st);
@@ -393,6 +402,111 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
         if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
     }
     return release(S, st, rc);
+}
+
+int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t *keep,
+                int32_t *steps, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (K < 1) return fail(CMFW_EINVAL, "screen length %d < 1", K);
+    if (!keep) return fail(CMFW_EINVAL, "keep is NULL");
+    if (in->n_traj == 0) return CMFW_OK;
+    cmfw::WalkArgs A;
+    SnapStore S;
+    fill_args(A, desc, in, K, 0, nullptr, S);
+    hipError_t e = cmfw::launch_screen(desc->rank, A, K, keep, steps, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "screen launch");
+}
+
+int64_t cmfw_mlp_param_count(void) { return (int64_t)cmfw::mlp_param_count(); }
+int64_t cmfw_mlp_param_bf16_count(void) { return (int64_t)cmfw::mlp_param_bf16_count(); }
+size_t cmfw_mlp_workspace_bytes(int32_t batch) {
+    return batch > 0 ? cmfw::mlp_workspace_bytes(batch) : 0;
+}
+
+int cmfw_mlp_forward(const float *params, const uint16_t *params_bf16, const uint16_t *X,
+                     int32_t batch, float *y, void *workspace, size_t workspace_bytes,
+                     void *stream) {
+    if (!params || !params_bf16 || !X || !y || batch < 0)
+        return fail(CMFW_EINVAL, "mlp_forward: null pointer or negative batch");
+    if (batch == 0) return CMFW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t need = cmfw::mlp_workspace_bytes(batch);
+    void *ws = workspace, *owned = nullptr;
+    if (ws && workspace_bytes < need)
+        return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes, need);
+    if (!ws) {
+        hipError_t e = hipMallocAsync(&owned, need, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(mlp)");
+        ws = owned;
+    }
+    hipError_t e = cmfw::mlp_forward(params, params_bf16, X, batch, y, ws, st);
+    int rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_forward");
+    if (owned) (void)hipFreeAsync(owned, st);
+    return rc;
+}
+
+int cmfw_mlp_grad(const float *params, const uint16_t *params_bf16, const uint16_t *X,
+                  const float *target, int32_t batch, float *grad, float *sq_rows,
+                  void *workspace, size_t workspace_bytes, void *stream) {
+    if (!params || !params_bf16 || !X || !target || !grad || batch <= 0)
+        return fail(CMFW_EINVAL, "mlp_grad: null pointer or empty batch");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t need = cmfw::mlp_workspace_bytes(batch);
+    void *ws = workspace, *owned = nullptr;
+    if (ws && workspace_bytes < need)
+        return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes, need);
+    if (!ws) {
+        hipError_t e = hipMallocAsync(&owned, need, st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(mlp)");
+        ws = owned;
+    }
+    hipError_t e = cmfw::mlp_grad(params, params_bf16, X, target, batch, grad, sq_rows, ws, st);
+    int rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_grad");
+    if (owned) (void)hipFreeAsync(owned, st);
+    return rc;
+}
+
+int cmfw_mlp_adam(float *params, uint16_t *params_bf16, const float *grad, float *m, float *v,
+                  int32_t step, float lr, void *stream) {
+    if (!params || !params_bf16 || !grad || !m || !v || step < 1)
+        return fail(CMFW_EINVAL, "mlp_adam: null pointer or step < 1");
+    hipError_t e = cmfw::mlp_adam(params, params_bf16, grad, m, v, step, lr, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_adam");
+}
+
+int cmfw_mlp_sync_bf16(const float *params, uint16_t *params_bf16, void *stream) {
+    if (!params || !params_bf16) return fail(CMFW_EINVAL, "mlp_sync_bf16: null pointer");
+    hipError_t e = cmfw::mlp_sync_bf16(params, params_bf16, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_sync_bf16");
+}
+
+int cmfw_mlp_features(const cmfw_desc *desc, const cmfw_inputs *in, const cmfw_results *prefix,
+                      int32_t K, uint16_t *X, void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (!prefix || !X || K < 1 || !prefix->valid || !prefix->n_fwd || !prefix->n_inv ||
+        !prefix->log_scale || !prefix->delta || !prefix->precision || !prefix->limit ||
+        !prefix->n_limits || !prefix->p || !prefix->q)
+        return fail(CMFW_EINVAL, "mlp_features: prefix results incomplete");
+    if (in->n_traj == 0) return CMFW_OK;
+    cmfw::WalkArgs A;
+    SnapStore S;
+    cmfw_results r = *prefix;
+    fill_args(A, desc, in, K, 0, &r, S);
+    hipError_t e = cmfw::mlp_features(A, K, X, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_features");
+}
+
+int cmfw_mlp_gemm(int32_t mode, int32_t M, int32_t N, int32_t K, const uint16_t *A, int32_t lda,
+                  const uint16_t *B, int32_t ldb, float *C, int32_t ldc, void *stream) {
+    if (mode < 0 || mode > 3 || M < 0 || N < 0 || K < 0 || !A || !B || !C)
+        return fail(CMFW_EINVAL, "mlp_gemm: bad arguments");
+    // the kernel's transposed-operand modes index A as [K][lda], B as [N][ldb]
+    const int tb = mode & 1, ta = (mode >> 1) & 1;
+    const int kmode = ta * 2 + tb;
+    hipError_t e = cmfw::mlp_gemm_test(kmode, M, N, K, A, lda, B, ldb, C, ldc, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "mlp_gemm");
 }
 
 int cmfw_set_tuning(const char *key, int64_t value) {
@@ -409,6 +523,18 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "min_segment_steps")) {
         if (value < 1 || value > 1000000) return fail(CMFW_EINVAL, "min_segment_steps out of range");
         g_min_segment_steps = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "fast_segments")) {
+        if (value < 1 || value > cmfw::kMaxSegments)
+            return fail(CMFW_EINVAL, "fast_segments outside 1..%d", cmfw::kMaxSegments);
+        g_fast_segments = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "rns_min_segment_steps")) {
+        if (value < 1 || value > 1000000)
+            return fail(CMFW_EINVAL, "rns_min_segment_steps out of range");
+        g_rns_min_segment_steps = (int)value;
         return CMFW_OK;
     }
     return fail(CMFW_EINVAL, "unknown tuning key %s", key);

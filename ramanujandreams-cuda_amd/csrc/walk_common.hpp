@@ -36,12 +36,22 @@ struct WalkArgs {
     int prefix;  // > 0: walk only this many trajectory steps (screening)
 };
 
-template <typename T, int K>
-__device__ __forceinline__ T pick(const T (&a)[K], int idx) {
-    T r = a[0];
+// a[idx] for a wave-uniform idx, without indexed access: a select chain
+// written as (j == idx) ? a[j] : r is folded by the compiler back into an
+// indexed load from a private array, which then lives in scratch memory.
+// A bit-mask merge keeps every element in its VGPRs.
+__device__ __forceinline__ int32_t pick(const int32_t (&a)[16], int idx) {
+    int32_t r = 0;
 #pragma unroll
-    for (int j = 1; j < K; ++j) r = (j == idx) ? a[j] : r;
+    for (int j = 0; j < 16; ++j) r |= a[j] & -(int32_t)(j == idx);
     return r;
+}
+__device__ __forceinline__ double pick(const double (&a)[8], int idx) {
+    unsigned long long r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        r |= (unsigned long long)__double_as_longlong(a[j]) & (0ull - (unsigned long long)(j == idx));
+    return __longlong_as_double((long long)r);
 }
 
 __device__ __forceinline__ double snap_at(const WalkArgs &A, int D, int s, int i, int j, int t) {

@@ -25,6 +25,8 @@ inline uint32_t host_modulus(int k) {
     return (k >= 0 && k < kMaxRnsModuli) ? t[k] : 0u;
 }
 hipError_t launch_walk_f64_ref(int D, const WalkArgs &A, hipStream_t st);
+hipError_t launch_screen(int D, const WalkArgs &A, int K, int32_t *keep, int32_t *steps,
+                         hipStream_t st);
 hipError_t launch_walk_f64_fast(int D, const WalkArgs &A, const int *bounds, int nseg,
                                 void *seg_ws, hipStream_t st);
 size_t fast_segment_bytes(int D, int n, int nseg);
@@ -33,8 +35,23 @@ hipError_t launch_walk_f32x2(int D, const WalkArgs &A, const int *bounds, int ns
 hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, uint64_t *crt,
                            const int32_t *only, hipStream_t st);
 size_t rns_seg_bytes(int D, int n, int np, int nseg);
-int rns_plan(int N, int h, int *b, int maxseg);
-hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds, int nseg,
-                               void *ws, uint32_t *resid, uint64_t *crt, int32_t **fallback,
-                               hipStream_t st);
+int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, int *b,
+             uint8_t *crt, int maxseg);
+hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
+                               const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
+                               uint64_t *crt, int32_t **fallback, hipStream_t st);
+// proposal network (mlp.hip)
+size_t mlp_param_count();
+size_t mlp_param_bf16_count();
+size_t mlp_workspace_bytes(int B);
+hipError_t mlp_forward(const float *P, const uint16_t *Pb, const uint16_t *X, int B, float *y,
+                       void *ws, hipStream_t st);
+hipError_t mlp_grad(const float *P, const uint16_t *Pb, const uint16_t *X, const float *target,
+                    int B, float *grad, float *sq_out, void *ws, hipStream_t st);
+hipError_t mlp_adam(float *P, uint16_t *Pb, const float *grad, float *m, float *v, int step,
+                    float lr, hipStream_t st);
+hipError_t mlp_sync_bf16(const float *P, uint16_t *Pb, hipStream_t st);
+hipError_t mlp_features(const WalkArgs &A, int K, uint16_t *X, hipStream_t st);
+hipError_t mlp_gemm_test(int mode, int M, int N, int K, const uint16_t *A, int lda,
+                         const uint16_t *B, int ldb, float *C, int ldc, hipStream_t st);
 }  // namespace cmfw

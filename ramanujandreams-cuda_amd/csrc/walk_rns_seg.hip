@@ -37,7 +37,8 @@ struct RnsSegArgs {
     WalkArgs w;
     int np, G;
     int nseg;
-    int b[kMaxRnsSegments + 1];  // segment boundaries = CRT points
+    int b[kMaxRnsSegments + 1];  // segment boundaries; every CRT point is one
+    uint8_t crtp[kMaxRnsSegments + 1];  // 1 where b[i] is a CRT point
     RnsConsts C;
     const uint32_t *dtab;   // [np][2*kDenTab+1] Montgomery a^(m-2) of a mod m
     uint32_t *segP;         // [nseg][np][D][D][n]
@@ -269,6 +270,8 @@ __global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
 #pragma unroll
             for (int j = 0; j < D; ++j) W[r][j] = row[j];
         }
+        // a boundary inside a CRT interval only splits the product
+        if (!RA.crtp[s + 1]) continue;
         // CRT point: rns_to_double, snapshot, re-encode (cuda/cmf_walk_rns.cu:285-296)
         const int step = RA.b[s + 1];
         uint32_t Wn[D][D];
@@ -361,30 +364,63 @@ size_t rns_seg_bytes(int D, int n, int np, int nseg) {
            align256(nn * 4) + align256((size_t)np * (2 * kDenTab + 1) * 4);
 }
 
-// CRT points of an RNS walk of depth N: multiples of 100 and the
-// checkpoints N-2h, N-h, N inside 1..N (cuda/cmf_walk_rns.cu:285).  Returns
-// the segment count (0 when N <= 0: no step is walked), or -1 when more
-// than maxseg segments would be needed.
-int rns_plan(int N, int h, int *b, int maxseg) {
-    int cnt = 0;
-    b[cnt++] = 0;
+// Segment plan of an RNS walk of depth N.  The CRT points are the multiples
+// of 100 and the checkpoints N-2h, N-h, N inside 1..N
+// (cuda/cmf_walk_rns.cu:285); each interval between two of them is split
+// evenly into pieces of at most L steps, L chosen so that n trajectories x
+// G modulus lanes x segments come near target_lanes (at least min_steps
+// steps a piece).  crt[i] marks the boundaries that are CRT points.
+// Returns the segment count (0 when N <= 0: no step is walked), or -1 when
+// the CRT points alone exceed maxseg.
+int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, int *b,
+             uint8_t *crt, int maxseg) {
+    b[0] = 0;
+    crt[0] = 0;
     if (N <= 0) return 0;
+    int cp[kMaxRnsSegments + 2];
+    int ncp = 0;
     const int cps[3] = {N - 2 * h, N - h, N};
     for (int step = 1; step <= N;) {
         int nxt = ((step + 99) / 100) * 100;
         for (int c = 0; c < 3; ++c)
             if (cps[c] >= step && cps[c] < nxt) nxt = cps[c];
         if (nxt > N) nxt = N;
-        if (cnt > maxseg) return -1;
-        b[cnt++] = nxt;
+        if (ncp >= maxseg) return -1;
+        cp[ncp++] = nxt;
         step = nxt + 1;
     }
-    return cnt - 1;
+    int G = 1;
+    while (G < np) G <<= 1;
+    const int64_t per_seg = (int64_t)(n > 0 ? n : 1) * G;
+    const int64_t want = (target_lanes + per_seg - 1) / per_seg;
+    int L = (int)((N + want - 1) / (want > 0 ? want : 1));
+    if (L < min_steps) L = min_steps;
+    if (L < 1) L = 1;
+    for (;;) {
+        int cnt = 1, prev = 0;
+        bool fits = true;
+        for (int c = 0; c < ncp && fits; ++c) {
+            const int len = cp[c] - prev;
+            const int pieces = (len + L - 1) / L;
+            for (int k = 1; k <= pieces; ++k) {
+                if (cnt > maxseg) {
+                    fits = false;
+                    break;
+                }
+                b[cnt] = prev + (int)(((int64_t)len * k) / pieces);
+                crt[cnt] = (k == pieces) ? 1 : 0;
+                ++cnt;
+            }
+            prev = cp[c];
+        }
+        if (fits) return cnt - 1;
+        L *= 2;
+    }
 }
 
-hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds, int nseg,
-                               void *ws, uint32_t *resid, uint64_t *crt, int32_t **fallback,
-                               hipStream_t st) {
+hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
+                               const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
+                               uint64_t *crt, int32_t **fallback, hipStream_t st) {
     RnsSegArgs RA;
     RA.w = A;
     RA.np = np;
@@ -392,7 +428,10 @@ hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *boun
     while (G < np) G <<= 1;
     RA.G = G;
     RA.nseg = nseg;
-    for (int i = 0; i <= nseg; ++i) RA.b[i] = bounds[i];
+    for (int i = 0; i <= nseg; ++i) {
+        RA.b[i] = bounds[i];
+        RA.crtp[i] = crtp[i];
+    }
     for (int k = 0; k < np; ++k) RA.C.mods[k] = make_mont(host_modulus(k));
     const size_t nn = (size_t)A.n;
     char *p = (char *)ws;

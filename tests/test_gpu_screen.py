@@ -1,0 +1,64 @@
+"""GPU checks of cmfw_screen, the position-only candidate screen: it never
+drops a trajectory whose first K steps the reference walk survives, and it
+keeps a trajectory the reference drops only where the failure is a
+matrix-state check (rare, but for the nilpotent steps of rank-2 walks)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("p,q,z", [(3, 2, 0.5), (4, 3, 0.5), (6, 5, 1.0), (8, 7, 1.0),
+                                   (5, 4, 1.0), (3, 2, 1.0)])
+def test_screen_against_reference_prefix(engine, p, q, z):
+    rng = np.random.default_rng(31 * p + int(4 * z))
+    n = 20000
+    D = rng.integers(-2, 4, (n, p + q))
+    D[D.sum(1) == 0, 0] = 1
+    S = rng.integers(1, 9, (n, p + q)).astype(float)
+    S[: n // 4] = [1.0] * p + [2.0] * q          # zeta starts
+    print(f"screen {p}F{q} z={z}", flush=True)
+    for K in (1, 3):
+        keep = engine.screen(p, q, z, D, S, K).cpu().numpy()
+        pre = engine.results_to_numpy(engine.walk_prefix_f64(p, q, z, D, S, K))
+        valid = pre["valid"] == 1
+        assert np.all(keep[valid] == 1), "screen dropped a surviving trajectory"
+        # kept but failed: a matrix-state check (the product vanished or
+        # overflowed).  Rare, except on rank-2 z=1 walks (3F2 at z=1),
+        # whose steps are often nilpotent: ~1.4% there.
+        extra = int(((keep == 1) & ~valid).sum())
+        assert extra <= (n // 20 if (p, z) == (3, 1.0) else max(2, n // 1000)), extra
+    k1 = engine.screen(p, q, z, D, S, 1).cpu().numpy()
+    k3 = engine.screen(p, q, z, D, S, 3).cpu().numpy()
+    assert np.all(k1[k3 == 1] == 1)
+
+
+def test_screen_candidate_workload(engine):
+    """8F7 z=1 zeta candidates (directions in {0..4}^15, mt19937_64(42)):
+    the screen keeps exactly the 2771 of 4,000,000 that survive
+    the reference's first step."""
+    import torch
+
+    p, q = 8, 7
+    D, S = engine.synth_inputs(42, 4_000_000, p, q, 1, 0, 0, 4)
+    dt = torch.from_numpy(D[:, :p + q].T.copy()).cuda()
+    st = torch.from_numpy(S[:, :p + q].T.copy()).cuda()
+    steps = torch.empty(dt.shape[1], dtype=torch.int32, device="cuda")
+    keep = engine.screen(p, q, 1.0, dt, st, 1, steps=steps)
+    pre = engine.walk_prefix_f64(p, q, 1.0, dt, st, 1)
+    assert int(keep.sum().item()) == 2771
+    assert torch.equal(keep, pre["valid"])
+    # every candidate evaluates at least its first axis step, survivors all
+    # of their |dir|_1 steps
+    s = steps.cpu().numpy()
+    assert s.min() >= 1
+    k = keep.cpu().numpy() == 1
+    assert np.array_equal(s[k], np.abs(D[k, :p + q]).sum(1))
+
+
+def test_screen_keeps_off_grid_starts(engine):
+    p, q = 4, 3
+    D = np.ones((8, p + q), int)
+    S = np.full((8, p + q), 1.5)
+    keep = engine.screen(p, q, 0.5, D, S, 1).cpu().numpy()
+    assert np.all(keep == 1)

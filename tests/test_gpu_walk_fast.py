@@ -94,21 +94,51 @@ def _with_counts(r, e):
     return out
 
 
-@pytest.mark.parametrize("n", [1, 7, 64, 3000])
-def test_fast_segment_plans(engine, orc, n):
-    """Different batch sizes give different depth-segment plans (many short
-    segments for few trajectories); results must not depend on the plan."""
+@pytest.mark.parametrize("nseg", [1, 8, 32, 64])
+def test_fast_segment_plans(engine, orc, nseg):
+    """Different depth-segment plans (one segment up to 64); results agree
+    with the reference order and the quad oracle whatever the plan."""
     p, q, z = 6, 5, 1.0
-    rng = np.random.default_rng(n)
+    n = 300
+    rng = np.random.default_rng(nseg)
     D = rng.integers(0, 3, (n, p + q))
     D[:, p:] = 0
     D[:, 0] = np.maximum(D[:, 0], 1)
     S = np.tile(np.array([1.0] * p + [2.0] * q), (n, 1))
     N, h = 400, 50
-    print(f"plans n={n}", flush=True)
-    g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
+    print(f"plans nseg={nseg}", flush=True)
+    engine.set_tuning("fast_segments", nseg)
+    try:
+        g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
+    finally:
+        engine.set_tuning("fast_segments", 32)
     e = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, N, h))
-    check(g, e, f"n={n}", orc.hp_walk_batch(p, q, z, D, S, N, h))
+    check(g, e, f"nseg={nseg}", orc.hp_walk_batch(p, q, z, D, S, N, h))
+
+
+@pytest.mark.parametrize("walk", ["walk_f64_fast", "walk_f32x2"])
+def test_fast_results_independent_of_batch(engine, walk):
+    """A trajectory's fast result is bit-identical whether it is walked
+    alone, in a small batch or inside a large one (the depth plan depends on
+    N and h only)."""
+    p, q, z = 8, 7, 1.0
+    rng = np.random.default_rng(5)
+    n = 5000
+    D = np.zeros((n, p + q), np.int64)
+    D[:, :p] = rng.integers(0, 5, (n, p))
+    D[:, p:] = rng.integers(0, 2, (n, q))
+    D[D.sum(1) == 0, 0] = 1
+    S = np.tile(np.array([1.0] * p + [2.0] * q), (n, 1))
+    f = getattr(engine, walk)
+    big = engine.results_to_numpy(f(p, q, z, D, S, 300, 50))
+    pick = np.nonzero(big["valid"] == 1)[0][:40]
+    if pick.size == 0:
+        pick = np.arange(40)
+    small = engine.results_to_numpy(f(p, q, z, D[pick], S[pick], 300, 50))
+    one = engine.results_to_numpy(f(p, q, z, D[pick[:1]], S[pick[:1]], 300, 50))
+    for fld in big.dtype.names:
+        assert np.array_equal(small[fld], big[fld][pick], equal_nan=True), fld
+        assert np.array_equal(one[fld], big[fld][pick[:1]], equal_nan=True), fld
 
 
 def test_fast_zeta_records(engine, published):
