@@ -57,12 +57,15 @@ def parse():
     ap.add_argument("--n-cand", type=int, default=4_000_000)
     ap.add_argument("--top-frac", type=float, default=0.01)
     ap.add_argument("--n-primes", type=int, default=8)
-    ap.add_argument("--deep-m", type=int, default=300, help="proposal: candidates walked deep")
+    ap.add_argument("--deep-m", type=int, default=400, help="proposal: candidates walked deep")
     ap.add_argument("--verify-v", type=int, default=128, help="proposal: candidates RNS-verified")
     ap.add_argument("--train-pairs", type=int, default=1_000_000)
     ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (5 times) and "
+                         "exit: the reference kernel for calibrating HBM byte counters")
     return ap.parse_args()
 
 
@@ -221,6 +224,16 @@ def main():
     import cmfwalk as cw
     from cmfwalk import proposal as pr
 
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib << 20
+        a = torch.empty(nb, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb}))
+        return
     cw.load()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t_setup = time.perf_counter()

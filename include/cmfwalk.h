@@ -172,6 +172,42 @@ int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t st
                       int32_t start_hi, int32_t dir_lo, int32_t dir_hi, int32_t *dir_out,
                       double *start_out);
 
+/* Mixed batches ----------------------------------------------------------
+ * Trajectories of different CMF (p, q), z and depth N in one submission
+ * (the reference runs one CMF, z and N per process: run_benchmark in
+ * cuda/cmf_walk_f64.cu:157-316).  cmfw_batch_create plans once: rank
+ * buckets, device work queues ordered longest walk first, an optional
+ * autotune of each bucket's launch shape, and one captured hipGraph (one
+ * fork/join branch per rank, a persistent kernel pulling 64-trajectory
+ * tiles from its queue).  cmfw_batch_run replays the graph on a stream.
+ * Walks use the fast FP64 engine without depth segments (same validity
+ * decisions as the reference; trajectories it cannot decide are walked in
+ * the reference's order).  Creation blocks the host (allocation, upload,
+ * tuning); runs do not. */
+typedef struct cmfw_batch_spec {
+    int32_t n_traj;
+    int32_t ld;           /* leading dimension of dir/start */
+    const int32_t *p;     /* host [n_traj] */
+    const int32_t *q;     /* host [n_traj] */
+    const double *z;      /* host [n_traj] */
+    const int32_t *N;     /* host [n_traj], depth of each walk */
+    int32_t h;            /* snapshot spacing: snapshots at N-2h, N-h, N */
+    const int32_t *dir;   /* device [16][ld], axes x1..xp, y1..yq of each trajectory */
+    const double *start;  /* device [16][ld] */
+} cmfw_batch_spec;
+
+typedef struct cmfw_batch cmfw_batch;
+
+/* out: device results [n_traj] (every field; n_fwd/n_inv may be NULL).
+ * tune != 0: time the candidate launch shapes per rank on a sample first
+ * (the choice is also remembered for later batches of this process). */
+int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int32_t tune,
+                      cmfw_batch **batch);
+int cmfw_batch_run(cmfw_batch *batch, void *stream);
+/* JSON description: buckets, their sizes and launch shapes */
+int cmfw_batch_info(const cmfw_batch *batch, char *buf, size_t len);
+void cmfw_batch_destroy(cmfw_batch *batch);
+
 /* Candidate screen: keep[t] = 0 when trajectory t certainly fails within
  * its first K trajectory steps, decided by the position-determined checks of
  * the walk (lead coefficient of the companion column, step denominator,
@@ -191,10 +227,14 @@ int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t
  *                       0 walk every trajectory sequentially;
  *   "target_lanes"      lanes an RNS walk aims to fill (262144):
  *                       trajectories x modulus lanes x segments;
- *   "fast_segments"     depth segments of a fast f64 / two-float walk (32);
+ *   "fast_segments"     depth segments of a fast f64 / two-float walk (64);
  *                       the plan depends on N and h only, so a trajectory's
  *                       result does not depend on the batch it is walked in;
  *   "min_segment_steps" shortest fast f64 / two-float depth segment (8);
+ *   "collapse_log2"     fast engines hand a rank <= 4 trajectory to the
+ *                       reference order when a product's max falls below
+ *                       2^-value of the terms that formed it (exact
+ *                       cancellation of nilpotent steps) (40);
  *   "rns_min_segment_steps" shortest RNS segment between two boundaries
  *                       (4); every CRT point is a boundary.
  * No counterpart in the reference. */

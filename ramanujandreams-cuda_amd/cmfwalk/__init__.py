@@ -68,6 +68,11 @@ SIGNATURES = {
                                        C.POINTER(Results), C.c_void_p]),
     "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "cmfw_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Results), C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    "cmfw_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cmfw_batch_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
+    "cmfw_batch_destroy": (None, [C.c_void_p]),
     "cmfw_screen": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_void_p,
                               C.c_void_p, C.c_void_p]),
     "cmfw_mlp_param_count": (C.c_int64, []),
@@ -284,6 +289,64 @@ def screen(p, q, z, dirs, starts, K=1, keep=None, steps=None, stream=None):
     _check(L.cmfw_screen(C.byref(d), C.byref(_inputs_struct(dt, st)), K, keep.data_ptr(),
                          None if steps is None else steps.data_ptr(), _stream_ptr(stream)))
     return keep
+
+
+class BatchSpec(C.Structure):
+    _fields_ = [("n_traj", C.c_int32), ("ld", C.c_int32), ("p", C.c_void_p), ("q", C.c_void_p),
+                ("z", C.c_void_p), ("N", C.c_void_p), ("h", C.c_int32), ("dir", C.c_void_p),
+                ("start", C.c_void_p)]
+
+
+class Batch:
+    """A planned mixed batch (cmfw_batch_*): per-trajectory p, q, z, N on the
+    host, directions/starts [n, 16] (host or device); results on the device.
+    run() replays the captured graph; info() describes buckets and shapes."""
+
+    def __init__(self, p, q, z, N, h, dirs, starts, tune=False):
+        import torch
+
+        L = load()
+        self._L = L
+        n = len(p)
+        self.p = np.ascontiguousarray(p, dtype=np.int32)
+        self.q = np.ascontiguousarray(q, dtype=np.int32)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.N = np.ascontiguousarray(N, dtype=np.int32)
+        d = torch.as_tensor(np.asarray(dirs), dtype=torch.int32)
+        s = torch.as_tensor(np.asarray(starts), dtype=torch.float64)
+        self.dir = torch.zeros((16, n), dtype=torch.int32, device="cuda")
+        self.start = torch.zeros((16, n), dtype=torch.float64, device="cuda")
+        self.dir[:d.shape[1]] = d.T.cuda()
+        self.start[:s.shape[1]] = s.T.cuda()
+        self.out = alloc_results(n)
+        spec = BatchSpec(n, n, self.p.ctypes.data, self.q.ctypes.data, self.z.ctypes.data,
+                         self.N.ctypes.data, h, self.dir.data_ptr(), self.start.data_ptr())
+        torch.cuda.synchronize()
+        self._h = C.c_void_p()
+        _check(L.cmfw_batch_create(C.byref(spec), C.byref(_results_struct(self.out)),
+                                   1 if tune else 0, C.byref(self._h)))
+
+    def run(self, stream=None):
+        _check(self._L.cmfw_batch_run(self._h, _stream_ptr(stream)))
+        return self.out
+
+    def info(self):
+        import json
+
+        buf = C.create_string_buffer(8192)
+        _check(self._L.cmfw_batch_info(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self._h:
+            self._L.cmfw_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def synth_inputs(seed, n, p, q, start_lo, start_hi, dir_lo, dir_hi):

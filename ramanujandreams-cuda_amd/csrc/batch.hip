@@ -1,0 +1,330 @@
+// batch.hip -- host planner of mixed batches (cmfw_batch_*).
+//
+// A batch holds trajectories of different CMF (p, q), z and depth N.  At
+// creation the planner
+//   1. validates every trajectory and computes its rank D (make_cmf_desc's
+//      rule, cuda/cmf_pfq.cuh:41-61);
+//   2. buckets the trajectories by D and orders each bucket by depth,
+//      longest first, into a device work queue;
+//   3. optionally autotunes each bucket's launch shape (block size and
+//      resident waves per SIMD of the persistent kernel) by timing the
+//      candidates on a strided sample of the bucket;
+//   4. captures one hipGraph: a fill kernel clears the tile counters, then
+//      one fork/join branch per non-empty bucket runs k_walk_batch<D> and
+//      k_walk_batch_exact<D> on its own stream, so buckets of different rank
+//      overlap on the GPU.
+// cmfw_batch_run replays the graph on the caller's stream.  All device
+// memory is allocated at creation and released by cmfw_batch_destroy.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/cmfwalk.h"
+#include "walk_common.hpp"
+#include "walk_kernels.hpp"
+
+namespace cmfw {
+float collapse_floor();
+int set_error(int code, const char *msg);
+int set_hip_error(hipError_t e, const char *what);
+}  // namespace cmfw
+
+namespace {
+
+constexpr int kMinD = 2, kMaxD = 8, kBuckets = kMaxD - kMinD + 1;
+
+struct Shape {
+    int block = 256;
+    int waves_per_simd = 1;
+    float tuned_ms = -1.f;  // sample time of the chosen shape (-1: not tuned)
+};
+
+// Process-wide table of the best shape found per rank (read by batches that
+// are created without tuning).
+Shape g_shape[kBuckets];
+bool g_shape_tuned[kBuckets] = {false};
+
+int rank_of(int p, int q, double z) {
+    int r = (p > q + 1) ? p : q + 1;
+    if (z == 1.0 && p == q + 1) r -= 1;
+    return r;
+}
+
+}  // namespace
+
+struct cmfw_batch {
+    int n = 0, h = 0, ld = 0, n_cu = 0;
+    cmfw::WalkArgs w{};
+    int32_t *bp = nullptr, *bq = nullptr, *bN = nullptr, *queue = nullptr, *counters = nullptr,
+            *exact = nullptr;
+    double *bz = nullptr, *snap = nullptr, *snap_ls = nullptr;
+    int off[kBuckets + 1] = {0};  // bucket b occupies queue[off[b], off[b+1])
+    Shape shape[kBuckets];
+    hipStream_t side[kBuckets] = {nullptr};
+    hipEvent_t fork = nullptr, join[kBuckets] = {nullptr};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &v) {
+    hipError_t e = hipMalloc((void **)dst, std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+int grid_of(const cmfw_batch *b, const Shape &s) {
+    const long long waves = (long long)b->n_cu * 4 * s.waves_per_simd;
+    const int per_block = s.block / 64;
+    return (int)std::max<long long>(1, waves / per_block);
+}
+
+hipError_t launch_bucket(cmfw_batch *b, int k, const int32_t *queue, int count, hipStream_t st) {
+    const Shape &s = b->shape[k];
+    return cmfw::launch_walk_batch(k + kMinD, b->w, b->bp, b->bq, b->bz, b->bN, b->h, queue,
+                                   count, b->counters + k, b->exact, s.block, grid_of(b, s), st);
+}
+
+// Times the candidate shapes of bucket k on a strided sample of its queue
+// (the sample's walks write results that the real run overwrites).
+hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
+    const int count = b->off[k + 1] - b->off[k];
+    if (count == 0) return hipSuccess;
+    const int want = std::min(count, b->n_cu * 64);
+    std::vector<int32_t> host(count);
+    hipError_t e = hipMemcpy(host.data(), b->queue + b->off[k], count * sizeof(int32_t),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    std::vector<int32_t> sample;
+    const double stride = (double)count / want;
+    for (int i = 0; i < want; ++i) sample.push_back(host[(size_t)(i * stride)]);
+    int32_t *dq = nullptr;
+    if ((e = upload(&dq, sample)) != hipSuccess) return e;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    Shape best;
+    float best_ms = 1e30f;
+    const int blocks[3] = {64, 128, 256};
+    const int waves[3] = {1, 2, 4};
+    for (int bi = 0; bi < 3 && e == hipSuccess; ++bi)
+        for (int wi = 0; wi < 3 && e == hipSuccess; ++wi) {
+            Shape s;
+            s.block = blocks[bi];
+            s.waves_per_simd = waves[wi];
+            b->shape[k] = s;
+            float ms = 1e30f;
+            for (int rep = 0; rep < 1 && e == hipSuccess; ++rep) {
+                if ((e = cmfw::launch_zero_i32(b->counters + k, 1, st)) != hipSuccess) break;
+                (void)hipEventRecord(e0, st);
+                if ((e = launch_bucket(b, k, dq, want, st)) != hipSuccess) break;
+                (void)hipEventRecord(e1, st);
+                if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+                float t = 0.f;
+                (void)hipEventElapsedTime(&t, e0, e1);
+                ms = std::min(ms, t);
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = s;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dq);
+    best.tuned_ms = best_ms;
+    b->shape[k] = best;
+    g_shape[k] = best;
+    g_shape_tuned[k] = true;
+    return e;
+}
+
+hipError_t capture(cmfw_batch *b) {
+    hipStream_t cs;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess) return e;
+    for (int k = 0; k < kBuckets; ++k) {
+        if (b->off[k + 1] == b->off[k]) continue;
+        if ((e = hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking)) != hipSuccess) return e;
+        if ((e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming)) != hipSuccess) return e;
+    }
+    if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+    hipError_t le = cmfw::launch_zero_i32(b->counters, kBuckets, cs);
+    if (le == hipSuccess) le = hipEventRecord(b->fork, cs);
+    for (int k = 0; k < kBuckets && le == hipSuccess; ++k) {
+        const int count = b->off[k + 1] - b->off[k];
+        if (count == 0) continue;
+        le = hipStreamWaitEvent(b->side[k], b->fork, 0);
+        if (le == hipSuccess) le = launch_bucket(b, k, b->queue + b->off[k], count, b->side[k]);
+        if (le == hipSuccess) le = hipEventRecord(b->join[k], b->side[k]);
+        if (le == hipSuccess) le = hipStreamWaitEvent(cs, b->join[k], 0);
+    }
+    e = hipStreamEndCapture(cs, &b->graph);
+    if (le != hipSuccess) e = le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cmfw_batch_destroy(cmfw_batch *b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int k = 0; k < kBuckets; ++k) {
+        if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    void *bufs[] = {b->bp, b->bq, b->bN, b->queue, b->counters, b->exact, b->bz, b->snap,
+                    b->snap_ls};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    delete b;
+}
+
+int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int32_t tune,
+                      cmfw_batch **handle) {
+    if (!handle) return cmfw::set_error(CMFW_EINVAL, "handle is NULL");
+    *handle = nullptr;
+    if (!spec || !out) return cmfw::set_error(CMFW_EINVAL, "spec/out is NULL");
+    const int n = spec->n_traj;
+    if (n < 0 || spec->ld < n) return cmfw::set_error(CMFW_EINVAL, "bad n_traj / ld");
+    if (n > 0 && (!spec->p || !spec->q || !spec->z || !spec->N || !spec->dir || !spec->start))
+        return cmfw::set_error(CMFW_EINVAL, "spec arrays are NULL");
+    if (n > 0 && (!out->valid || !out->limit || !out->delta || !out->p || !out->q ||
+                  !out->precision || !out->log_scale || !out->n_limits))
+        return cmfw::set_error(CMFW_EINVAL, "batch results need every field but n_fwd/n_inv");
+    std::vector<std::vector<int32_t>> bucket(kBuckets);
+    for (int t = 0; t < n; ++t) {
+        const int p = spec->p[t], q = spec->q[t];
+        if (p < 1 || q < 0 || p > 8 || q > 8 || p + q > CMFW_MAX_AXES) {
+            char msg[96];
+            snprintf(msg, sizeof msg, "trajectory %d: unsupported CMF %dF%d", t, p, q);
+            return cmfw::set_error(CMFW_EINVAL, msg);
+        }
+        const int D = rank_of(p, q, spec->z[t]);
+        if (D < kMinD || D > kMaxD) {
+            char msg[96];
+            snprintf(msg, sizeof msg, "trajectory %d: rank %d outside 2..8", t, D);
+            return cmfw::set_error(CMFW_EINVAL, msg);
+        }
+        bucket[D - kMinD].push_back(t);
+    }
+    cmfw_batch *b = new cmfw_batch;
+    b->n = n;
+    b->h = spec->h;
+    b->ld = spec->ld;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    b->n_cu = (e == hipSuccess) ? prop.multiProcessorCount : 256;
+    std::vector<int32_t> queue;
+    for (int k = 0; k < kBuckets; ++k) {
+        std::stable_sort(bucket[k].begin(), bucket[k].end(),
+                         [&](int32_t a, int32_t c) { return spec->N[a] > spec->N[c]; });
+        b->off[k] = (int)queue.size();
+        queue.insert(queue.end(), bucket[k].begin(), bucket[k].end());
+        b->shape[k] = g_shape_tuned[k] ? g_shape[k] : Shape();
+        b->shape[k].tuned_ms = -1.f;
+    }
+    b->off[kBuckets] = (int)queue.size();
+    std::vector<int32_t> vp(spec->p, spec->p + n), vq(spec->q, spec->q + n), vN(spec->N, spec->N + n);
+    std::vector<double> vz(spec->z, spec->z + n);
+    if (e == hipSuccess) e = upload(&b->bp, vp);
+    if (e == hipSuccess) e = upload(&b->bq, vq);
+    if (e == hipSuccess) e = upload(&b->bN, vN);
+    if (e == hipSuccess) e = upload(&b->bz, vz);
+    if (e == hipSuccess) e = upload(&b->queue, queue);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->counters, kBuckets * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->exact, std::max(n, 1) * sizeof(int32_t));
+    // snapshots: each bucket indexes [3][D][D][n] with its own D; buckets
+    // hold disjoint trajectories, so one buffer sized for D = 8 serves all
+    if (e == hipSuccess) e = hipMalloc((void **)&b->snap, (size_t)3 * 64 * std::max(n, 1) * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->snap_ls, (size_t)3 * std::max(n, 1) * 8);
+    if (e != hipSuccess) {
+        cmfw_batch_destroy(b);
+        return cmfw::set_hip_error(e, "cmfw_batch_create: device setup");
+    }
+    cmfw::WalkArgs &w = b->w;
+    memset(&w, 0, sizeof w);
+    w.n = n;
+    w.ld = spec->ld;
+    w.dir = spec->dir;
+    w.start = spec->start;
+    w.snap = b->snap;
+    w.snap_ls = b->snap_ls;
+    w.sld = std::max(n, 1);
+    w.collapse = cmfw::collapse_floor();
+    w.limit = out->limit;
+    w.delta = out->delta;
+    w.rp = out->p;
+    w.rq = out->q;
+    w.precision = out->precision;
+    w.log_scale = out->log_scale;
+    w.valid = out->valid;
+    w.n_limits = out->n_limits;
+    w.n_fwd = out->n_fwd;
+    w.n_inv = out->n_inv;
+    if (tune) {
+        hipStream_t ts;
+        e = hipStreamCreateWithFlags(&ts, hipStreamNonBlocking);
+        for (int k = 0; k < kBuckets && e == hipSuccess; ++k) e = tune_bucket(b, k, ts);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts);
+        (void)hipStreamDestroy(ts);
+        if (e != hipSuccess) {
+            cmfw_batch_destroy(b);
+            return cmfw::set_hip_error(e, "cmfw_batch_create: autotune");
+        }
+    }
+    if (n > 0 && (e = capture(b)) != hipSuccess) {
+        cmfw_batch_destroy(b);
+        return cmfw::set_hip_error(e, "cmfw_batch_create: graph capture");
+    }
+    *handle = b;
+    return CMFW_OK;
+}
+
+int cmfw_batch_run(cmfw_batch *b, void *stream) {
+    if (!b) return cmfw::set_error(CMFW_EINVAL, "batch is NULL");
+    if (b->n == 0) return CMFW_OK;
+    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    return (e == hipSuccess) ? CMFW_OK : cmfw::set_hip_error(e, "cmfw_batch_run");
+}
+
+int cmfw_batch_info(const cmfw_batch *b, char *buf, size_t len) {
+    if (!b || !buf || len == 0) return cmfw::set_error(CMFW_EINVAL, "bad arguments");
+    std::string s = "{\"n\": " + std::to_string(b->n) + ", \"cus\": " + std::to_string(b->n_cu) +
+                    ", \"buckets\": [";
+    bool first = true;
+    for (int k = 0; k < kBuckets; ++k) {
+        const int count = b->off[k + 1] - b->off[k];
+        if (count == 0) continue;
+        char item[256];
+        snprintf(item, sizeof item,
+                 "%s{\"rank\": %d, \"count\": %d, \"block\": %d, \"waves_per_simd\": %d, "
+                 "\"grid\": %d, \"tuned_sample_ms\": %.4f}",
+                 first ? "" : ", ", k + kMinD, count, b->shape[k].block,
+                 b->shape[k].waves_per_simd, grid_of(b, b->shape[k]), b->shape[k].tuned_ms);
+        s += item;
+        first = false;
+    }
+    s += "]}";
+    if (s.size() + 1 > len) return cmfw::set_error(CMFW_EINVAL, "buffer too small");
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return CMFW_OK;
+}
+
+}  // extern "C"

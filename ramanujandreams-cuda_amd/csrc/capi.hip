@@ -61,8 +61,14 @@ size_t snap_bytes(int D, int n) {
 // x 4 waves.  Fewer trajectories than that are split along their depth.
 int64_t g_target_lanes = 262144;
 int g_min_segment_steps = 8;
-int g_fast_segments = 32;
+int g_fast_segments = 64;
+// A fast walk over more trajectories than this runs chunk by chunk (same
+// plan, same results): the segment products need only a chunk's worth of
+// workspace.
+constexpr int kFastChunk = 131072;
 int g_rns_min_segment_steps = 4;
+// fast engines: collapse floor 2^-g_collapse_log2 (see keep_in_range)
+int g_collapse_log2 = 40;
 bool g_rns_segments = true;
 
 // Depth segments of a fast walk: multiples of a segment length L plus the
@@ -159,6 +165,7 @@ void fill_args(cmfw::WalkArgs &A, const cmfw_desc *d, const cmfw_inputs *in, int
     A.snap = S.m;
     A.snap_ls = S.ls;
     A.sld = S.ld;
+    A.collapse = ldexpf(1.f, -g_collapse_log2);
     if (out) {
         A.limit = out->limit;
         A.delta = out->delta;
@@ -188,6 +195,13 @@ const uint32_t kHostModuli[CMFW_MAX_PRIMES] = {
 
 }  // namespace
 
+namespace cmfw {
+float collapse_floor() { return ldexpf(1.f, -g_collapse_log2); }
+// error reporting for the other host files of the library
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+int set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
+}  // namespace cmfw
+
 extern "C" {
 
 const char *cmfw_last_error(void) { return g_err; }
@@ -212,7 +226,8 @@ size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, in
     if (!desc || n_traj < 0) return 0;
     int b[cmfw::kMaxSegments + 1];
     const int nseg = plan_segments(N, h, b);
-    size_t need = cmfw::fast_segment_bytes(desc->rank, n_traj, nseg);
+    size_t need = cmfw::fast_segment_bytes(desc->rank, n_traj < kFastChunk ? n_traj : kFastChunk,
+                                           nseg);
     if (n_primes >= 1 && n_primes <= CMFW_MAX_PRIMES) {
         int rb[cmfw::kMaxRnsSegments + 1];
         uint8_t rc[cmfw::kMaxRnsSegments + 1];
@@ -256,9 +271,10 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
     if (in->n_traj == 0) return CMFW_OK;
     hipStream_t st = (hipStream_t)stream;
     const int n = in->n_traj;
+    const int chunk = n < kFastChunk ? n : kFastChunk;
     int b[cmfw::kMaxSegments + 1];
     const int nseg = plan_segments(N, h, b);
-    const size_t seg_need = cmfw::fast_segment_bytes(desc->rank, n, nseg);
+    const size_t seg_need = cmfw::fast_segment_bytes(desc->rank, chunk, nseg);
     SnapStore S;
     void *seg = nullptr, *owned = nullptr;
     if (workspace) {
@@ -276,8 +292,24 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
     }
     cmfw::WalkArgs A;
     fill_args(A, desc, in, N, h, out, S);
-    hipError_t e = ff ? cmfw::launch_walk_f32x2(desc->rank, A, b, nseg, seg, st)
-                      : cmfw::launch_walk_f64_fast(desc->rank, A, b, nseg, seg, st);
+    hipError_t e = hipSuccess;
+    for (int t0 = 0; t0 < n && e == hipSuccess; t0 += chunk) {
+        // trajectories [t0, t0 + cnt): the same SoA arrays, offset by t0
+        cmfw::WalkArgs C = A;
+        C.n = (n - t0 < chunk) ? n - t0 : chunk;
+        C.dir = A.dir + t0;
+        C.start = A.start + t0;
+        C.snap = A.snap + t0;
+        C.snap_ls = A.snap_ls + t0;
+        double **dp[] = {&C.limit, &C.delta, &C.rp, &C.rq, &C.precision, &C.log_scale};
+        for (double **f : dp)
+            if (*f) *f += t0;
+        int32_t **ip[] = {&C.valid, &C.n_limits, &C.n_fwd, &C.n_inv};
+        for (int32_t **f : ip)
+            if (*f) *f += t0;
+        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, st)
+               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, st);
+    }
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, ff ? "walk_f32x2 launch" : "walk_f64_fast launch");
     if (owned) {
         hipError_t e2 = hipFreeAsync(owned, st);
@@ -523,6 +555,11 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "min_segment_steps")) {
         if (value < 1 || value > 1000000) return fail(CMFW_EINVAL, "min_segment_steps out of range");
         g_min_segment_steps = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "collapse_log2")) {
+        if (value < 8 || value > 120) return fail(CMFW_EINVAL, "collapse_log2 outside 8..120");
+        g_collapse_log2 = (int)value;
         return CMFW_OK;
     }
     if (!strcmp(key, "fast_segments")) {

@@ -17,7 +17,7 @@
 namespace cmfw {
 
 constexpr double kEps = 2.220446049250313e-16;
-constexpr int kMaxSegments = 64;
+constexpr int kMaxSegments = 80;
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -34,6 +34,9 @@ struct WalkArgs {
     double *limit, *delta, *rp, *rq, *precision, *log_scale;
     int32_t *valid, *n_limits, *n_fwd, *n_inv;
     int prefix;  // > 0: walk only this many trajectory steps (screening)
+    // fast engines: a product whose max falls below collapse x (the scale of
+    // the terms that formed it) is treated as vanished (see keep_in_range)
+    float collapse;
 };
 
 // a[idx] for a wave-uniform idx, without indexed access: a select chain

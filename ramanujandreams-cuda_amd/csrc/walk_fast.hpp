@@ -112,14 +112,21 @@ __device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], con
 }
 
 // Keeps max |M| inside [2^-R, 2^R] (R = Ar::kRangeLog2) with exact
-// power-of-two rescales.  Returns false when the max is zero or not finite:
-// the reference's normalize_projective (cuda/cmf_pfq.cuh:109) would then
-// fail -- or, where this arithmetic cancelled exactly (small integers, exact
-// FMA, power-of-two scaling) but the reference's inexact 1/max scaling left a
-// rounding residue, carry on.  The caller hands such a trajectory to the
-// reference-order walk, which decides it exactly as the reference does.
+// power-of-two rescales.  Returns false when the max is zero or not finite,
+// or when it fell below `floor` times `terms` (a bound of the terms whose
+// sums formed the entries): the product vanished in exact arithmetic
+// (nilpotent steps) and what is left is rounding noise, and whether the
+// reference's unfused operation order leaves an exact 0.0
+// (normalize_projective fails, cuda/cmf_pfq.cuh:109) or a residue it carries
+// on with is decided by its rounding, not by this arithmetic's.  The caller
+// hands such a trajectory to the reference-order walk, which decides it
+// exactly as the reference does.  On success `mxo` is the max after
+// rescaling.
+constexpr int kCollapseMaxRank = 4;
+
 template <int D, class Ar>
-__device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2, bool force) {
+__device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2, bool force,
+                                              double terms, float floor, typename Ar::M &mxo) {
     using Mg = typename Ar::M;
     Mg mx = 0;
 #pragma unroll
@@ -127,6 +134,12 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
 #pragma unroll
         for (int c = 0; c < D; ++c) mx = fmax(mx, Ar::mag(M[r][c]));
     if (!(mx > 0) || !isfinite(mx)) return false;
+    // Only ranks <= 4 are checked: there nilpotent steps make the
+    // reference's exact 0.0 a real outcome (3F2 and 4F3 at z = 1).  At higher
+    // rank a collapse to rounding noise is common (most surviving 8F7 z = 1
+    // zeta walks have one) but the reference carries on with its residue,
+    // as this arithmetic does: validity agrees without a handoff.
+    if (D <= kCollapseMaxRank && (double)mx < terms * (double)floor) return false;
     const Mg hi = (Mg)ldexp(1.0, Ar::kRangeLog2), lo = (Mg)ldexp(1.0, -Ar::kRangeLog2);
     if (force || mx > hi || mx < lo) {
         const int e = ilogb((double)mx) + 1;
@@ -135,14 +148,19 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
 #pragma unroll
             for (int c = 0; c < D; ++c) M[r][c] = Ar::scale2(M[r][c], -e);
         e2 += e;
+        mx = (Mg)ldexp((double)mx, -e);
     }
+    mxo = mx;
     return true;
 }
 
+// One axis step M <- M * (I + C/a) or M * (I + C/a)^-1.  `tsc` receives a
+// bound of the step matrix's entries (max |M| times tsc bounds the terms
+// of the product's entries).
 template <int D, class Ar>
 __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const double (&xs)[8],
                                                const double (&ys)[8], int p, int q, double z,
-                                               int ax, int sign, bool &inverse) {
+                                               int ax, int sign, bool &inverse, double &tsc) {
     using T = typename Ar::T;
     double xv[8], yv[8], denom;
     inverse = axis_eval(xs, ys, p, ax, sign, xv, yv, denom);
@@ -150,6 +168,11 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
     if (!companion_column_fast<D, Ar>(xv, yv, p, q, z, col)) return false;
     if (denom == 0.0) return false;  // |denom| <= 64 eps for an integer point
     const T ia = Ar::recip(Ar::from_d(denom));
+    double scol = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) scol += fabs(Ar::to_d(col[k]));
+    const double aia = fabs(1.0 / denom);
+    tsc = fmax(1.0, aia) * fmax(1.0, scol * aia);
     if (!inverse) {
 #pragma unroll
         for (int r = 0; r < D; ++r) {
@@ -174,7 +197,6 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
         dabs += fabs(Ar::to_d(be[k]) * Ar::to_d(col[k]));
     }
     T sd = Ar::fma(db, ia, Ar::one());
-    const double aia = fabs(1.0 / denom);
     // The fast sd decides only when it is far from the reference's threshold
     // 128 eps max(1, |db/a|): more than 2^20 times the size of its own terms.
     bool neg = false;
@@ -203,6 +225,13 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
     }
     if (!Ar::finite(sd)) return false;
     const T isd = Ar::recip(sd);
+    {
+        // entries of the inverse step: powers of 1/a up to D-1, over sd
+        double pw = 1.0;
+#pragma unroll
+        for (int j = 0; j + 1 < D; ++j) pw *= fmax(1.0, aia);
+        tsc = pw * fmax(1.0, scol) / fmin(1.0, fabs(Ar::to_d(sd)));
+    }
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         T al[D];
@@ -255,6 +284,7 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
 #pragma unroll
     for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
     int e2 = 0, nf = 0, ni = 0;
+    typename Ar::M mx = 1;  // max of the identity
     bool ok = true;
     const int steps = S.b[s + 1] - S.b[s];
     for (int st = 0; ok && st < steps; ++st) {
@@ -264,12 +294,13 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 bool inverse;
-                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse);
+                double tsc;
+                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse, tsc);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
-                if (!keep_in_range<D, Ar>(M, e2, false)) {
-                    S.segV[o] = 2;  // zero or non-finite product: reference order decides
+                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx)) {
+                    S.segV[o] = 2;  // product vanished or overflowed: reference order decides
                     S.segF[o] = nf;
                     S.segI[o] = ni;
                     return;
@@ -282,7 +313,7 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     S.segI[o] = ni;
     S.segV[o] = ok ? 1 : 0;
     if (!ok) return;
-    if (!keep_in_range<D, Ar>(M, e2, true)) {
+    if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) {
         S.segV[o] = 2;
         return;
     }
@@ -325,10 +356,15 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
             break;
         }
         T G[D][D];
+        typename Ar::M gmx = 0, wmx = 0;
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int j = 0; j < D; ++j) G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * A.n + t];
+            for (int j = 0; j < D; ++j) {
+                G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * A.n + t];
+                gmx = fmax(gmx, Ar::mag(G[i][j]));
+                wmx = fmax(wmx, Ar::mag(W[i][j]));
+            }
         // W <- W * G, one row at a time
 #pragma unroll
         for (int r = 0; r < D; ++r) {
@@ -344,7 +380,9 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
             for (int j = 0; j < D; ++j) W[r][j] = row[j];
         }
         e2 += S.segE[o];
-        if (!keep_in_range<D, Ar>(W, e2, true)) {
+        // the product of two segments can vanish across their boundary
+        typename Ar::M mxo;
+        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, A.collapse, mxo)) {
             walk_exact_traj<D>(A, t);
             return;
         }
@@ -381,6 +419,75 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
     return hipGetLastError();
+}
+
+// The whole walk of trajectory t in one lane, without depth segments: for
+// batches that fill the GPU by themselves.  The snapshots are taken as the
+// combine kernel takes them (FP64 max-normalisation, log of the scale once
+// per snapshot).  Returns false, writing nothing, when the trajectory must
+// be walked in the reference's order instead (start off the integer grid,
+// or a product that vanished or overflowed).
+template <int D, class Ar>
+__device__ bool walk_full_fast(const WalkArgs &A, int t) {
+    using T = typename Ar::T;
+    int32_t dir[16];
+    double xs[8], ys[8];
+    load_traj(A, t, dir, xs, ys);
+    bool clean = true;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) clean = clean && fast_point(xs[j]) && fast_point(ys[j]);
+    if (!clean) return false;
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    bool ok = !(N <= 0 || h <= 0 || N - 2 * h <= 0);
+    T M[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    int e2 = 0, nf = 0, ni = 0;
+    typename Ar::M mx = 1;  // max of the identity
+    for (int step = 1; ok && step <= N; ++step) {
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                bool inverse;
+                double tsc;
+                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse, tsc);
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                if (!ok) break;
+                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                    return false;
+                axis_move(xs, ys, A.p, ax, sign);
+            }
+        }
+        if (!ok) break;
+        const int s = (step == cp1) ? 0 : (step == cp2) ? 1 : (step == N) ? 2 : -1;
+        if (s >= 0) {
+            double smx = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j) smx = fmax(smx, fabs(Ar::to_d(M[i][j])));
+            const double inv = 1.0 / smx;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    A.snap[((size_t)((s * D + i) * D + j)) * A.sld + t] = Ar::to_d(M[i][j]) * inv;
+            A.snap_ls[(size_t)s * A.sld + t] = (double)e2 * 0.6931471805599453 + log(smx);
+        }
+    }
+    if (A.n_fwd) A.n_fwd[t] = nf;
+    if (A.n_inv) A.n_inv[t] = ni;
+    finish_from_snapshots<D>(A, t, ok);
+    return true;
 }
 
 template <class Ar>

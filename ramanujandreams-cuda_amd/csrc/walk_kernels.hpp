@@ -6,7 +6,7 @@
 namespace cmfw {
 struct WalkArgs;
 constexpr int kMaxRnsModuli = 64;
-constexpr int kMaxRnsSegments = 128;
+constexpr int kMaxRnsSegments = 256;
 
 // The reference's modulus table (cuda/cmf_walk_rns.cu:26-43), host side.
 inline uint32_t host_modulus(int k) {
@@ -40,6 +40,12 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
                                uint64_t *crt, int32_t **fallback, hipStream_t st);
+// mixed batches (walk_batch.hip)
+hipError_t launch_walk_batch(int D, const WalkArgs &w, const int32_t *bp, const int32_t *bq,
+                             const double *bz, const int32_t *bN, int h, const int32_t *queue,
+                             int count, int32_t *counter, int32_t *exact, int block, int grid,
+                             hipStream_t st);
+hipError_t launch_zero_i32(int32_t *p, int n, hipStream_t st);
 // proposal network (mlp.hip)
 size_t mlp_param_count();
 size_t mlp_param_bf16_count();

@@ -40,8 +40,15 @@ def check(g, r, tag, hp=None):
         return
     v &= hp["ok"] == 1
     exact = hp["ratio"][:, 2]
-    e_g = _rel(g["p"] / g["q"], exact)[v]
-    e_r = _rel(r["p"] / r["q"], exact)[v]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        e_g = _rel(g["p"] / g["q"], exact)[v]
+        e_r = _rel(r["p"] / r["q"], exact)[v]
+    # rank-2 walks can end with q exactly 0 (p/q infinite) in both orders
+    fin_g, fin_r = np.isfinite(e_g), np.isfinite(e_r)
+    assert (fin_g == fin_r).mean() >= 0.98, (tag, "finite p/q")
+    e_g, e_r = e_g[fin_g & fin_r], e_r[fin_g & fin_r]
+    if e_g.size == 0:
+        return
     # compare the error distributions (on ill-conditioned walks both errors
     # are amplified rounding noise, so walk-by-walk ratios are random)
     q_g = np.percentile(e_g, [50, 90, 100])
@@ -65,7 +72,8 @@ def check(g, r, tag, hp=None):
 
 
 @pytest.mark.parametrize("p,q,z", [(3, 2, 0.5), (4, 3, 0.5), (5, 4, 1.0), (6, 5, 1.0),
-                                   (6, 5, 0.5), (8, 7, 1.0), (8, 7, 0.5), (5, 2, 0.5)])
+                                   (6, 5, 0.5), (8, 7, 1.0), (8, 7, 0.5), (5, 2, 0.5),
+                                   (3, 2, 1.0), (4, 3, 1.0)])
 def test_fast_matches_reference_order(engine, orc, p, q, z):
     rng = np.random.default_rng(31 * p + int(8 * z))
     n = 500
