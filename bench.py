@@ -40,6 +40,19 @@ sys.path.insert(0, PKG)
 
 P, Q, Z, N_DEPTH, H = 8, 7, 1.0, 1000, 50
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak, AMD spec
+# RNS: modular products per second.  MI355X: 256 CUs x 2.4 GHz x 16 lanes
+# per clock of quarter-rate 32-bit multiplies (v_mad_u64_u32 /
+# v_mul_lo_u32), three of them per Montgomery product.
+RNS_PEAK_TMODMUL = 256 * 2.4e9 * 16 / 3 / 1e12
+FERMAT_MODMUL = 45  # a^(m-2) by square-and-multiply, 30-bit exponent
+
+
+def rns_modmuls(d, n_fwd, n_inv, n_primes):
+    """Modular products of the reference's RNS algorithm per lane and axis
+    step: forward 2d^2 + one inverse, inverse d(5d+1) + (d+1) inverses, an
+    inverse being a Fermat power."""
+    return n_primes * (n_fwd * (2 * d * d + FERMAT_MODMUL) +
+                       n_inv * (d * (5 * d + 1) + (d + 1) * FERMAT_MODMUL))
 
 
 def canonical_flops(d, p, q):
@@ -63,6 +76,11 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["search", "batch"], default="search",
+                    help="search: the headline 8F7 search; batch: the mixed-batch walk "
+                         "(3F2..8F7, z in {0.5,1}, N in [200,5000], h=50)")
+    ap.add_argument("--batch-n", type=int, default=262144)
+    ap.add_argument("--no-tune", action="store_true", help="batch: skip the autotuner")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times) and "
                          "exit: the reference kernel for calibrating HBM byte counters")
@@ -172,6 +190,17 @@ def timed_steps(torch, fn, steps, warmup):
     return out, (time.perf_counter() - t0) / steps, T
 
 
+def load_traffic():
+    """HBM bytes per launch of the walk kernels, from the rocprofv3 FETCH_SIZE
+    and WRITE_SIZE passes summarised in profiles/hbm_traffic.json (already
+    scaled by the copy-kernel calibration); empty when absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return {k: v["bytes_per_launch"] for k, v in json.load(f)["kernels"].items()}
+
+
 class _NoTimer:
     def mark(self, name):
         pass
@@ -213,6 +242,60 @@ def cpu_baseline(D, S, sample, top_frac, n_primes):
             "survivors": int(alive.size)}
 
 
+def run_batch(args, torch, cw, np):
+    """Mixed batch: trajectories of 3F2..8F7 at z in {0.5, 1} with depths in
+    [200, 5000] (h=50), starts in {1..8}, directions in {0..3}, all walked
+    by one replay of the batch's captured graph."""
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    p, q, z, N, D, S = cw.synth_mixed(42, args.batch_n)
+    t0 = time.perf_counter()
+    b = cw.Batch(p, q, z, N, 50, D, S, tune=not args.no_tune)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    info = b.info()
+    log(f"batch planned in {plan_s:.1f}s: " + json.dumps(info))
+    for i in range(args.warmup):
+        b.run()
+        torch.cuda.synchronize()
+        log(f"warmup {i} done")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e0.record()
+    for i in range(args.steps):
+        out = b.run()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t1) / args.steps
+    ms = e0.elapsed_time(e1) / args.steps
+    r = cw.results_to_numpy(out)
+    rank = np.array([cw.make_desc(int(a), int(c), float(x)).rank for a, c, x in zip(p, q, z)])
+    flops = 0.0
+    for d in np.unique(rank):
+        for pp in np.unique(p[rank == d]):
+            m = (rank == d) & (p == pp)
+            f, i = canonical_flops(int(d), int(pp), int(pp) - 1)
+            flops += float(r["n_fwd"][m].astype(np.float64).sum() * f +
+                           r["n_inv"][m].astype(np.float64).sum() * i)
+    ach = flops / (ms * 1e-3) / 1e12
+    line = {"metric": "mixed-batch traj/s (3F2..8F7, z in {0.5,1}, N in [200,5000], h=50)",
+            "value": args.batch_n / (wall), "unit": "traj/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{args.batch_n} trajectories, mt19937_64(42) draw order "
+                                   "cmf, z, N, start, dir; one graph replay per step",
+                       "parallelism": "single-gpu"},
+            "graph_ms": ms, "plan_s": plan_s, "buckets": info["buckets"],
+            "valid": int(r["valid"].sum()),
+            "roofline": {"kernel": "k_walk_batch<D> (all ranks)", "bound": "valu",
+                         "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                         "algorithmic_work_per_launch": flops, "launch_ms": ms},
+            "cpu_baseline": None}
+    b.close()
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -235,6 +318,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": nb}))
         return
     cw.load()
+    if args.workload == "batch":
+        return run_batch(args, torch, cw, np)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t_setup = time.perf_counter()
     srch = Search(torch, cw, pr, args.n_cand, args.n_primes)
@@ -310,14 +395,26 @@ def main():
     kern = {"screen": "k_screen<7>", "deep": "k_walk_seg<7,f64> + combine",
             "features": "k_walk_seg<7,f64> + combine (100 steps)"}
     dom = max(("screen", "features", "deep", "verify"), key=lambda k: st_p[k])
-    fl = flops.get(dom)
-    ach = fl / (st_p[dom] * 1e-3) / 1e12 if fl else None
-    roofline = {"kernel": kern.get(dom, "k_rns_segment<7> + k_rns_combine<7>"),
-                "stage": dom, "bound": "valu",
-                "achieved": ach, "peak": FP64_PEAK_TFLOPS if fl else None,
-                "unit": "TFLOP/s" if fl else None,
-                "frac": ach / FP64_PEAK_TFLOPS if fl else None, "traffic": None,
-                "algorithmic_flops_per_launch": fl, "launch_ms": st_p[dom]}
+    traffic = load_traffic()
+    if dom == "verify":
+        mm = float(rns_modmuls(7, rns_p["n_fwd"].astype(np.float64).sum(),
+                               rns_p["n_inv"].astype(np.float64).sum(), args.n_primes))
+        ach = mm / (st_p[dom] * 1e-3) / 1e12
+        roofline = {"kernel": "k_rns_segment<7> + k_rns_combine<7>", "stage": dom,
+                    "bound": "valu", "achieved": ach, "peak": RNS_PEAK_TMODMUL,
+                    "unit": "Tmodmul/s", "frac": ach / RNS_PEAK_TMODMUL,
+                    "traffic": traffic.get("k_rns_segment<7>"),
+                    "algorithmic_work_per_launch": mm, "launch_ms": st_p[dom]}
+    else:
+        fl = flops[dom]
+        ach = fl / (st_p[dom] * 1e-3) / 1e12
+        roofline = {"kernel": kern[dom], "stage": dom, "bound": "valu", "achieved": ach,
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                    "traffic": traffic.get(kern[dom].split(" ")[0]),
+                    "algorithmic_work_per_launch": fl, "launch_ms": st_p[dom]}
+    roofline["deep_f64"] = {"kernel": kern["deep"], "achieved": deep_flops / (st_p["deep"] * 1e-3) / 1e12,
+                            "frac": deep_flops / (st_p["deep"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                            "launch_ms": st_p["deep"]}
 
     value = args.n_cand / sec_p
     line = {

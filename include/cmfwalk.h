@@ -221,6 +221,18 @@ void cmfw_batch_destroy(cmfw_batch *batch);
 int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t *keep,
                 int32_t *steps, void *stream);
 
+/* Mixed synthetic batch (host arrays): per trajectory, in this draw order
+ * from std::mt19937_64(seed) with value = lo + rng() % (hi - lo + 1): the
+ * CMF pFq with p in cmf_lo..cmf_hi and q = p - 1, z = 1 if rng() % 2 else
+ * 0.5, N, the start point (p + q values) and the direction (p + q values;
+ * all zero becomes e_1).  dir_out/start_out are [n][16], trajectory-major
+ * like cmfw_synth_inputs.  No counterpart in the reference (one CMF per run
+ * there). */
+int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, int32_t N_lo,
+                     int32_t N_hi, int32_t start_lo, int32_t start_hi, int32_t dir_lo,
+                     int32_t dir_hi, int32_t *p_out, int32_t *q_out, double *z_out,
+                     int32_t *N_out, int32_t *dir_out, double *start_out);
+
 /* Process-wide tuning knobs (read when a walk is launched; not for
  * concurrent use with running calls).  Keys:
  *   "rns_segments"      1 (default) walk RNS depth segments in parallel,

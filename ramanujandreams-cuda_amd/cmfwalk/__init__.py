@@ -68,6 +68,8 @@ SIGNATURES = {
                                        C.POINTER(Results), C.c_void_p]),
     "cmfw_synth_inputs": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
+    "cmfw_synth_mixed": (C.c_int, [C.c_uint64, C.c_int64] + [C.c_int32] * 8 +
+                         [C.c_void_p] * 6),
     "cmfw_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Results), C.c_int32,
                                     C.POINTER(C.c_void_p)]),
     "cmfw_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -347,6 +349,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def synth_mixed(seed, n, cmf_lo=3, cmf_hi=8, N_lo=200, N_hi=5000, start_lo=1, start_hi=8,
+                dir_lo=0, dir_hi=3):
+    """Host arrays (p, q, z, N, dirs [n, 16], starts [n, 16]) of a mixed
+    batch from the library's mt19937_64 generator."""
+    L = load_library()
+    p = np.zeros(n, np.int32)
+    q = np.zeros(n, np.int32)
+    z = np.zeros(n, np.float64)
+    N = np.zeros(n, np.int32)
+    D = np.zeros((n, 16), np.int32)
+    S = np.zeros((n, 16), np.float64)
+    _check(L.cmfw_synth_mixed(seed, n, cmf_lo, cmf_hi, N_lo, N_hi, start_lo, start_hi, dir_lo,
+                              dir_hi, p.ctypes.data, q.ctypes.data, z.ctypes.data, N.ctypes.data,
+                              D.ctypes.data, S.ctypes.data))
+    return p, q, z, N, D, S
 
 
 def synth_inputs(seed, n, p, q, start_lo, start_hi, dir_lo, dir_hi):

@@ -97,7 +97,7 @@ hipError_t launch_bucket(cmfw_batch *b, int k, const int32_t *queue, int count, 
 hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
     const int count = b->off[k + 1] - b->off[k];
     if (count == 0) return hipSuccess;
-    const int want = std::min(count, b->n_cu * 64);
+    const int want = std::min(count, 2048);
     std::vector<int32_t> host(count);
     hipError_t e = hipMemcpy(host.data(), b->queue + b->off[k], count * sizeof(int32_t),
                              hipMemcpyDeviceToHost);
@@ -112,10 +112,10 @@ hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
     (void)hipEventCreate(&e1);
     Shape best;
     float best_ms = 1e30f;
-    const int blocks[3] = {64, 128, 256};
-    const int waves[3] = {1, 2, 4};
-    for (int bi = 0; bi < 3 && e == hipSuccess; ++bi)
-        for (int wi = 0; wi < 3 && e == hipSuccess; ++wi) {
+    const int blocks[2] = {64, 256};
+    const int waves[2] = {1, 2};
+    for (int bi = 0; bi < 2 && e == hipSuccess; ++bi)
+        for (int wi = 0; wi < 2 && e == hipSuccess; ++wi) {
             Shape s;
             s.block = blocks[bi];
             s.waves_per_simd = waves[wi];

@@ -381,6 +381,41 @@ int cmfw_synth_inputs(uint64_t seed, int64_t n, int32_t p, int32_t q, int32_t st
     return CMFW_OK;
 }
 
+int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, int32_t N_lo,
+                     int32_t N_hi, int32_t start_lo, int32_t start_hi, int32_t dir_lo,
+                     int32_t dir_hi, int32_t *p_out, int32_t *q_out, double *z_out,
+                     int32_t *N_out, int32_t *dir_out, double *start_out) {
+    if (n < 0 || cmf_lo < 3 || cmf_hi > 8 || cmf_lo > cmf_hi || N_lo > N_hi ||
+        start_lo > start_hi || dir_lo > dir_hi || !p_out || !q_out || !z_out || !N_out ||
+        !dir_out || !start_out)
+        return fail(CMFW_EINVAL, "bad synth_mixed arguments");
+    std::mt19937_64 rng(seed);
+    const uint64_t cspan = (uint64_t)(cmf_hi - cmf_lo + 1), nspan = (uint64_t)(N_hi - N_lo + 1);
+    const uint64_t sspan = (uint64_t)(start_hi - start_lo + 1), dspan = (uint64_t)(dir_hi - dir_lo + 1);
+    for (int64_t t = 0; t < n; ++t) {
+        const int p = cmf_lo + (int)(rng() % cspan);
+        p_out[t] = p;
+        q_out[t] = p - 1;
+        z_out[t] = (rng() % 2) ? 1.0 : 0.5;
+        N_out[t] = N_lo + (int32_t)(rng() % nspan);
+        int32_t *d = dir_out + t * 16;
+        double *st = start_out + t * 16;
+        for (int a = 0; a < 16; ++a) {
+            d[a] = 0;
+            st[a] = 0.0;
+        }
+        const int naxes = 2 * p - 1;
+        for (int a = 0; a < naxes; ++a) st[a] = (double)(start_lo + (int64_t)(rng() % sspan));
+        bool zero = true;
+        for (int a = 0; a < naxes; ++a) {
+            d[a] = dir_lo + (int32_t)(rng() % dspan);
+            zero = zero && d[a] == 0;
+        }
+        if (zero) d[0] = 1;
+    }
+    return CMFW_OK;
+}
+
 int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h,
                   int32_t n_primes, cmfw_results *out, cmfw_snapshots *snaps,
                   uint32_t *resid_out, uint64_t *crt_out, void *workspace,
