@@ -7,11 +7,12 @@
 //   2. buckets the trajectories by D and orders each bucket by depth,
 //      longest first, into a device work queue;
 //   3. optionally autotunes each bucket's launch shape (block size and
-//      resident waves per SIMD of the persistent kernel) by timing the
-//      candidates on a strided sample of the bucket;
-//   4. captures one hipGraph: a fill kernel clears the tile counters, then
-//      one fork/join branch per non-empty bucket runs k_walk_batch<D> and
-//      k_walk_batch_exact<D> on its own stream, so buckets of different rank
+//      resident waves per SIMD of the persistent segment kernel) by timing
+//      the candidates on the bucket itself;
+//   4. captures one hipGraph: a fill kernel clears the item counters, then
+//      one fork/join branch per non-empty bucket runs its chain (screen,
+//      compaction, persistent segment walk, combine, reference-order walks;
+//      walk_batch.hip) on its own stream, so buckets of different rank
 //      overlap on the GPU.
 // cmfw_batch_run replays the graph on the caller's stream.  All device
 // memory is allocated at creation and released by cmfw_batch_destroy.
@@ -29,6 +30,8 @@
 
 namespace cmfw {
 float collapse_floor();
+int fast_segments_target();
+int fast_min_segment_steps();
 int set_error(int code, const char *msg);
 int set_hip_error(hipError_t e, const char *what);
 }  // namespace cmfw
@@ -59,9 +62,10 @@ int rank_of(int p, int q, double z) {
 struct cmfw_batch {
     int n = 0, h = 0, ld = 0, n_cu = 0;
     cmfw::WalkArgs w{};
-    int32_t *bp = nullptr, *bq = nullptr, *bN = nullptr, *queue = nullptr, *counters = nullptr,
-            *exact = nullptr;
+    int32_t *bp = nullptr, *bq = nullptr, *bN = nullptr, *queue = nullptr, *counters = nullptr;
     double *bz = nullptr, *snap = nullptr, *snap_ls = nullptr;
+    void *ws[kBuckets] = {nullptr};  // per bucket: segments, survivors, flags
+    int target = 64, min_steps = 8;
     int off[kBuckets + 1] = {0};  // bucket b occupies queue[off[b], off[b+1])
     Shape shape[kBuckets];
     hipStream_t side[kBuckets] = {nullptr};
@@ -86,27 +90,32 @@ int grid_of(const cmfw_batch *b, const Shape &s) {
     return (int)std::max<long long>(1, waves / per_block);
 }
 
-hipError_t launch_bucket(cmfw_batch *b, int k, const int32_t *queue, int count, hipStream_t st) {
+hipError_t launch_bucket(cmfw_batch *b, int k, int part, hipStream_t st) {
     const Shape &s = b->shape[k];
-    return cmfw::launch_walk_batch(k + kMinD, b->w, b->bp, b->bq, b->bz, b->bN, b->h, queue,
-                                   count, b->counters + k, b->exact, s.block, grid_of(b, s), st);
+    cmfw::BatchLaunch L;
+    L.w = b->w;
+    L.bp = b->bp;
+    L.bq = b->bq;
+    L.bz = b->bz;
+    L.bN = b->bN;
+    L.h = b->h;
+    L.queue = b->queue + b->off[k];
+    L.count = b->off[k + 1] - b->off[k];
+    L.counter = b->counters + k;
+    L.ws = b->ws[k];
+    L.target = b->target;
+    L.min_steps = b->min_steps;
+    return cmfw::launch_walk_batch(k + kMinD, L, s.block, grid_of(b, s), part, st);
 }
 
-// Times the candidate shapes of bucket k on a strided sample of its queue
-// (the sample's walks write results that the real run overwrites).
+// Times the candidate shapes of bucket k's persistent segment kernel on
+// the bucket itself (after its screen; the timed runs' segment products are
+// recomputed by every real run).
 hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
     const int count = b->off[k + 1] - b->off[k];
     if (count == 0) return hipSuccess;
-    const int want = std::min(count, 2048);
-    std::vector<int32_t> host(count);
-    hipError_t e = hipMemcpy(host.data(), b->queue + b->off[k], count * sizeof(int32_t),
-                             hipMemcpyDeviceToHost);
+    hipError_t e = launch_bucket(b, k, 1, st);
     if (e != hipSuccess) return e;
-    std::vector<int32_t> sample;
-    const double stride = (double)count / want;
-    for (int i = 0; i < want; ++i) sample.push_back(host[(size_t)(i * stride)]);
-    int32_t *dq = nullptr;
-    if ((e = upload(&dq, sample)) != hipSuccess) return e;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -124,7 +133,7 @@ hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
             for (int rep = 0; rep < 1 && e == hipSuccess; ++rep) {
                 if ((e = cmfw::launch_zero_i32(b->counters + k, 1, st)) != hipSuccess) break;
                 (void)hipEventRecord(e0, st);
-                if ((e = launch_bucket(b, k, dq, want, st)) != hipSuccess) break;
+                if ((e = launch_bucket(b, k, 2, st)) != hipSuccess) break;
                 (void)hipEventRecord(e1, st);
                 if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
                 float t = 0.f;
@@ -138,7 +147,6 @@ hipError_t tune_bucket(cmfw_batch *b, int k, hipStream_t st) {
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(dq);
     best.tuned_ms = best_ms;
     b->shape[k] = best;
     g_shape[k] = best;
@@ -163,7 +171,7 @@ hipError_t capture(cmfw_batch *b) {
         const int count = b->off[k + 1] - b->off[k];
         if (count == 0) continue;
         le = hipStreamWaitEvent(b->side[k], b->fork, 0);
-        if (le == hipSuccess) le = launch_bucket(b, k, b->queue + b->off[k], count, b->side[k]);
+        if (le == hipSuccess) le = launch_bucket(b, k, 7, b->side[k]);
         if (le == hipSuccess) le = hipEventRecord(b->join[k], b->side[k]);
         if (le == hipSuccess) le = hipStreamWaitEvent(cs, b->join[k], 0);
     }
@@ -187,10 +195,11 @@ void cmfw_batch_destroy(cmfw_batch *b) {
         if (b->join[k]) (void)hipEventDestroy(b->join[k]);
     }
     if (b->fork) (void)hipEventDestroy(b->fork);
-    void *bufs[] = {b->bp, b->bq, b->bN, b->queue, b->counters, b->exact, b->bz, b->snap,
-                    b->snap_ls};
+    void *bufs[] = {b->bp, b->bq, b->bN, b->queue, b->counters, b->bz, b->snap, b->snap_ls};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    for (int k = 0; k < kBuckets; ++k)
+        if (b->ws[k]) (void)hipFree(b->ws[k]);
     delete b;
 }
 
@@ -249,7 +258,12 @@ int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int3
     if (e == hipSuccess) e = upload(&b->bz, vz);
     if (e == hipSuccess) e = upload(&b->queue, queue);
     if (e == hipSuccess) e = hipMalloc((void **)&b->counters, kBuckets * sizeof(int32_t));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->exact, std::max(n, 1) * sizeof(int32_t));
+    b->target = cmfw::fast_segments_target();
+    b->min_steps = cmfw::fast_min_segment_steps();
+    for (int k = 0; k < kBuckets && e == hipSuccess; ++k) {
+        const int count = b->off[k + 1] - b->off[k];
+        if (count > 0) e = hipMalloc(&b->ws[k], cmfw::batch_bucket_bytes(k + kMinD, count));
+    }
     // snapshots: each bucket indexes [3][D][D][n] with its own D; buckets
     // hold disjoint trajectories, so one buffer sized for D = 8 serves all
     if (e == hipSuccess) e = hipMalloc((void **)&b->snap, (size_t)3 * 64 * std::max(n, 1) * 8);

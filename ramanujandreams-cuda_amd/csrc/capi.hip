@@ -82,27 +82,14 @@ int plan_segments(int N, int h, int *b) {
         b[1] = 0;
         return 1;  // the walk is invalid; one empty segment
     }
-    int L = (N + g_fast_segments - 1) / g_fast_segments;
-    if (L < g_min_segment_steps) L = g_min_segment_steps;
-    for (;;) {
-        int cnt = 0;
-        b[cnt++] = 0;
-        const int cps[3] = {N - 2 * h, N - h, N};
-        int next = L, ci = 0;
-        while (true) {
-            int v;
-            if (ci < 3 && (next > N || cps[ci] <= next)) v = cps[ci++];
-            else v = next;
-            if (v > b[cnt - 1]) {
-                if (cnt > cmfw::kMaxSegments) break;
-                b[cnt++] = v;
-            }
-            if (v == next) next += L;
-            if (b[cnt - 1] == N) break;
-        }
-        if (cnt <= cmfw::kMaxSegments + 1 && b[cnt - 1] == N) return cnt - 1;
-        L *= 2;
+    const cmfw::FastPlan P = cmfw::fast_plan(N, h, g_fast_segments, g_min_segment_steps);
+    int cnt = 0;
+    b[0] = 0;
+    for (int v = 0; v < N;) {
+        v = cmfw::fast_plan_next(P, v);
+        b[++cnt] = v;
     }
+    return cnt;
 }
 
 // Scratch snapshot storage: the caller's snaps, the caller's workspace, or
@@ -197,6 +184,8 @@ const uint32_t kHostModuli[CMFW_MAX_PRIMES] = {
 
 namespace cmfw {
 float collapse_floor() { return ldexpf(1.f, -g_collapse_log2); }
+int fast_segments_target() { return g_fast_segments; }
+int fast_min_segment_steps() { return g_min_segment_steps; }
 // error reporting for the other host files of the library
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 int set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }

@@ -2,17 +2,26 @@
 // CMF (p, q), z and depth N walked in one submission.
 //
 // The host planner (batch.hip) puts the trajectories of each rank D in a
-// device work queue, longest walks first.  k_walk_batch<D> is a persistent
-// kernel: each wave takes the next tile of 64 queue entries with one atomic
-// on the rank's counter and walks them, one trajectory per lane, in the
-// fast FP64 engine (walk_full_fast: no depth segments, the batch fills the
-// GPU by itself).  Longest-first order keeps the lanes of a wave on walks of
-// similar length and leaves the short walks to fill the tail.  Trajectories
-// the fast engine cannot decide are flagged and walked in the reference's
-// order by k_walk_batch_exact<D>, which carries that path's register
-// footprint alone.
+// device queue, longest walks first, and captures, per rank, this chain:
+//   k_batch_screen<D>   the position checks of the first trajectory step;
+//                       a trajectory that fails there is finished at once
+//                       (invalid, with the reference's executed-step counts)
+//   k_batch_compact     survivors' queue positions, in queue order
+//   k_batch_seg<D>      persistent: waves take 64 (survivor, depth segment)
+//                       items per atomic on the rank's counter and walk each
+//                       segment in the fast FP64 engine; the depth plan of a
+//                       trajectory depends on its N and h only (fast_plan),
+//                       so its result equals cmfw_walk_f64_fast's
+//   k_batch_combine<D>  one survivor per lane: segment products in order,
+//                       snapshots, limits and delta
+//   k_batch_exact<D>    survivors the fast engine cannot decide, walked in
+//                       the reference's order (that path's registers alone)
+// Depth segments matter here: most trajectories of a mixed batch die within
+// a few steps, and the few that live walk up to thousands of steps; one lane
+// per walk would leave the GPU idle behind the longest.
 #include "walk_fast.hpp"
 #include "walk_kernels.hpp"
+#include "walk_screen.hpp"
 
 namespace cmfw {
 
@@ -22,10 +31,15 @@ struct BatchArgs {
     const double *bz;        // per trajectory z
     const int32_t *bN;       // per trajectory depth
     int h;
-    const int32_t *queue;    // trajectory indices of this rank
+    const int32_t *queue;    // trajectory indices of this rank, longest walk first
     int count;
-    int32_t *counter;        // tile counter, zero at launch
-    int32_t *exact;          // [n] 1 where the reference order must walk
+    int32_t *keep;           // [count] by queue position
+    int32_t *kept;           // [count] queue positions of the survivors, in queue order
+    int32_t *nkept;          // [1]
+    int32_t *counter;        // segment-item counter, zero at launch
+    int32_t *exact;          // [count] by survivor slot
+    SegStore<ArithF64> seg;  // slot = survivor slot, ld = count
+    int target, min_steps;   // fast plan parameters
 };
 
 __device__ __forceinline__ WalkArgs traj_args(const BatchArgs &B, int t) {
@@ -39,29 +53,100 @@ __device__ __forceinline__ WalkArgs traj_args(const BatchArgs &B, int t) {
     return A;
 }
 
+__device__ __forceinline__ bool walk_valid_depth(const WalkArgs &A) {
+    return !(A.N <= 0 || A.h <= 0 || A.N - 2 * A.h <= 0);
+}
+
 template <int D>
-__global__ __launch_bounds__(256) void k_walk_batch(BatchArgs B) {
+__global__ __launch_bounds__(256) void k_batch_screen(BatchArgs B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.count) return;
+    const int t = B.queue[i];
+    const WalkArgs A = traj_args(B, t);
+    int nf = 0, ni = 0;
+    const bool ok = walk_valid_depth(A) && screen_traj<D>(A, t, 1, nf, ni);
+    B.keep[i] = ok ? 1 : 0;
+    if (ok) return;
+    // the reference fails at this check: same counts, invalid result
+    if (A.n_fwd) A.n_fwd[t] = nf;
+    if (A.n_inv) A.n_inv[t] = ni;
+    finish_from_snapshots<D>(A, t, false);
+}
+
+// One block of 1024 threads: stable compaction of keep[] by wave ballots.
+__global__ __launch_bounds__(1024) void k_batch_compact(BatchArgs B) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < B.count; base += 1024) {
+        const int i = base + tid;
+        const bool k = i < B.count && B.keep[i] != 0;
+        const unsigned long long mask = __ballot(k);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(mask);
+        __syncthreads();
+        int off = carry;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (k) B.kept[off + before] = i;
+        __syncthreads();
+        if (tid == 0) {
+            int tot = 0;
+            for (int w = 0; w < 16; ++w) tot += wsum[w];
+            carry += tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *B.nkept = carry;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_batch_seg(BatchArgs B) {
     const int lane = threadIdx.x & 63;
+    const int nk = *B.nkept;
+    const long long total = (long long)kMaxSegments * nk;
     for (;;) {
-        int base = 0;
+        long long base = 0;
         if (lane == 0) base = atomicAdd(B.counter, 64);
         base = __shfl(base, 0, 64);
-        if (base >= B.count) break;  // every wave reaches this once the queue is drained
-        const int i = base + lane;
-        if (i < B.count) {
-            const int t = B.queue[i];
-            const WalkArgs A = traj_args(B, t);
-            B.exact[t] = walk_full_fast<D, ArithF64>(A, t) ? 0 : 1;
+        if (base >= total) break;  // every wave reaches this once the items are drained
+        const long long item = base + lane;
+        if (item >= total) continue;
+        const int s = (int)(item / nk), j = (int)(item % nk);
+        const int t = B.queue[B.kept[j]];
+        const WalkArgs A = traj_args(B, t);
+        const FastPlan P = fast_plan(A.N, A.h, B.target, B.min_steps);
+        int b0 = 0, b1 = fast_plan_next(P, 0);
+        for (int k = 0; k < s && b0 < A.N; ++k) {
+            b0 = b1;
+            b1 = fast_plan_next(P, b0);
         }
+        if (b0 >= A.N) continue;  // the plan has fewer segments
+        ArithF64::T M[D][D];
+        int e2, nf, ni;
+        const int st = walk_segment<D, ArithF64>(A, t, b0, b1 - b0, M, e2, nf, ni);
+        store_segment<D, ArithF64>(B.seg, s, (size_t)j, st, M, e2, nf, ni);
     }
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_walk_batch_exact(BatchArgs B) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B.count) return;
-    const int t = B.queue[i];
-    if (!B.exact[t]) return;
+__global__ __launch_bounds__(256) void k_batch_combine(BatchArgs B) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *B.nkept) return;
+    const int t = B.queue[B.kept[j]];
+    const WalkArgs A = traj_args(B, t);
+    const FastPlan P = fast_plan(A.N, A.h, B.target, B.min_steps);
+    const bool ok = combine_segments<D, ArithF64>(A, t, B.seg, (size_t)j, fast_plan_count(P),
+                                                  PlanBounds{P, 0});
+    B.exact[j] = ok ? 0 : 1;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_batch_exact(BatchArgs B) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *B.nkept || !B.exact[j]) return;
+    const int t = B.queue[B.kept[j]];
     walk_exact_traj<D>(traj_args(B, t), t);
 }
 
@@ -71,37 +156,76 @@ __global__ void k_zero_i32(int32_t *p, int n) {
 }
 
 template <int D>
-static hipError_t launch_batch_d(const BatchArgs &B, int block, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_walk_batch<D>, dim3(grid), dim3(block), 0, st, B);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_walk_batch_exact<D>, dim3((B.count + 255) / 256), dim3(256), 0, st, B);
-    return hipGetLastError();
+static hipError_t launch_batch_d(const BatchArgs &B, int block, int grid, int part,
+                                 hipStream_t st) {
+    const int g = (B.count + 255) / 256;
+    hipError_t e = hipSuccess;
+    if (part & 1) {
+        hipLaunchKernelGGL(k_batch_screen<D>, dim3(g), dim3(256), 0, st, B);
+        hipLaunchKernelGGL(k_batch_compact, dim3(1), dim3(1024), 0, st, B);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (part & 2) {
+        hipLaunchKernelGGL(k_batch_seg<D>, dim3(grid), dim3(block), 0, st, B);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (part & 4) {
+        hipLaunchKernelGGL(k_batch_combine<D>, dim3(g), dim3(256), 0, st, B);
+        hipLaunchKernelGGL(k_batch_exact<D>, dim3(g), dim3(256), 0, st, B);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
-hipError_t launch_walk_batch(int D, const WalkArgs &w, const int32_t *bp, const int32_t *bq,
-                             const double *bz, const int32_t *bN, int h, const int32_t *queue,
-                             int count, int32_t *counter, int32_t *exact, int block, int grid,
+size_t batch_bucket_bytes(int D, int count) {
+    const size_t n = (size_t)count;
+    return align256((size_t)kMaxSegments * D * D * n * 8) + 4 * align256((size_t)kMaxSegments * n * 4) +
+           4 * align256(n * 4) + align256(4);
+}
+
+hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, int part,
                              hipStream_t st) {
     BatchArgs B;
-    B.w = w;
-    B.bp = bp;
-    B.bq = bq;
-    B.bz = bz;
-    B.bN = bN;
-    B.h = h;
-    B.queue = queue;
-    B.count = count;
-    B.counter = counter;
-    B.exact = exact;
+    B.w = L.w;
+    B.bp = L.bp;
+    B.bq = L.bq;
+    B.bz = L.bz;
+    B.bN = L.bN;
+    B.h = L.h;
+    B.queue = L.queue;
+    B.count = L.count;
+    B.counter = L.counter;
+    B.target = L.target;
+    B.min_steps = L.min_steps;
+    // the bucket's workspace, carved as batch_bucket_bytes lays it out
+    const size_t n = (size_t)L.count;
+    char *p = (char *)L.ws;
+    B.seg.segM = (double *)p;
+    p += align256((size_t)kMaxSegments * D * D * n * 8);
+    B.seg.segE = (int32_t *)p;
+    p += align256((size_t)kMaxSegments * n * 4);
+    B.seg.segV = (int32_t *)p;
+    p += align256((size_t)kMaxSegments * n * 4);
+    B.seg.segF = (int32_t *)p;
+    p += align256((size_t)kMaxSegments * n * 4);
+    B.seg.segI = (int32_t *)p;
+    p += align256((size_t)kMaxSegments * n * 4);
+    B.seg.ld = n;
+    B.keep = (int32_t *)p;
+    p += align256(n * 4);
+    B.kept = (int32_t *)p;
+    p += align256(n * 4);
+    B.exact = (int32_t *)p;
+    p += align256(n * 4);
+    B.nkept = (int32_t *)p;
     switch (D) {
-        case 2: return launch_batch_d<2>(B, block, grid, st);
-        case 3: return launch_batch_d<3>(B, block, grid, st);
-        case 4: return launch_batch_d<4>(B, block, grid, st);
-        case 5: return launch_batch_d<5>(B, block, grid, st);
-        case 6: return launch_batch_d<6>(B, block, grid, st);
-        case 7: return launch_batch_d<7>(B, block, grid, st);
-        case 8: return launch_batch_d<8>(B, block, grid, st);
+        case 2: return launch_batch_d<2>(B, block, grid, part, st);
+        case 3: return launch_batch_d<3>(B, block, grid, part, st);
+        case 4: return launch_batch_d<4>(B, block, grid, part, st);
+        case 5: return launch_batch_d<5>(B, block, grid, part, st);
+        case 6: return launch_batch_d<6>(B, block, grid, part, st);
+        case 7: return launch_batch_d<7>(B, block, grid, part, st);
+        case 8: return launch_batch_d<8>(B, block, grid, part, st);
         default: return hipErrorInvalidValue;
     }
 }

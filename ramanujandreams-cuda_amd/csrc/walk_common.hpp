@@ -21,6 +21,46 @@ constexpr int kMaxSegments = 80;
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Depth-segment plan of the fast engines.  The boundaries are the
+// multiples of L and the snapshot depths N-2h, N-h, N; L is about
+// N / target (at least min_steps) and doubles until there are at most
+// kMaxSegments segments.  It depends on N and h only, so a trajectory's
+// fast result does not depend on the batch it is walked in.  Host and
+// device evaluate the same function.
+struct FastPlan {
+    int N, cp1, cp2, L;
+};
+
+// the first boundary after prev
+__host__ __device__ inline int fast_plan_next(const FastPlan &P, int prev) {
+    const int m = (prev / P.L + 1) * P.L;
+    int v = m > P.N ? P.N : m;
+    if (P.cp1 > prev && P.cp1 < v) v = P.cp1;
+    if (P.cp2 > prev && P.cp2 < v) v = P.cp2;
+    return v;
+}
+
+// valid walks only (N > 0, h > 0, N - 2h > 0)
+__host__ __device__ inline FastPlan fast_plan(int N, int h, int target, int min_steps) {
+    FastPlan P{N, N - 2 * h, N - h, 1};
+    int L = (N + target - 1) / target;
+    if (L < min_steps) L = min_steps;
+    if (L < 1) L = 1;
+    for (;;) {
+        P.L = L;
+        int cnt = 0;
+        for (int b = 0; b < N; b = fast_plan_next(P, b)) ++cnt;
+        if (cnt <= kMaxSegments) return P;
+        L *= 2;
+    }
+}
+
+__host__ __device__ inline int fast_plan_count(const FastPlan &P) {
+    int cnt = 0;
+    for (int b = 0; b < P.N; b = fast_plan_next(P, b)) ++cnt;
+    return cnt;
+}
+
 struct WalkArgs {
     int p, q, naxes, n, ld;
     double z;

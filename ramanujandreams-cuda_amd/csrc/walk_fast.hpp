@@ -248,34 +248,29 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
     return true;
 }
 
+// Walks trajectory steps (b0, b0 + steps] of trajectory t from the identity.
+// Returns 1 (ok: M, e2 hold the segment product), 0 (the walk fails in this
+// segment) or 2 (the reference order must decide: start off the integer
+// grid, or a product that vanished or overflowed).  nf / ni count the
+// executed forward / inverse axis steps, the failing one included.
 template <int D, class Ar>
-__global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
-    using T = typename Ar::T;
-    const WalkArgs &A = S.w;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (t >= A.n) return;
-    const size_t o = (size_t)s * A.n + t;
+__device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typename Ar::T (&M)[D][D],
+                            int &e2, int &nf, int &ni) {
     int32_t dir[16];
     double xs[8], ys[8];
     load_traj(A, t, dir, xs, ys);
+    nf = ni = e2 = 0;
     bool clean = true;
 #pragma unroll
     for (int j = 0; j < 8; ++j) clean = clean && fast_point(xs[j]) && fast_point(ys[j]);
-    if (!clean) {
-        S.segV[o] = 2;
-        S.segF[o] = 0;
-        S.segI[o] = 0;
-        return;
-    }
-    // start of this segment: start + b_s * dir (exact integers)
-    const double b0 = (double)S.b[s];
+    if (!clean) return 2;
+    // start of this segment: start + b0 * dir (exact integers)
+    const double fb0 = (double)b0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        xs[j] = fma(b0, (double)(j < A.p ? dir[j] : 0), xs[j]);
-        ys[j] = fma(b0, (double)(j < A.q ? pick(dir, A.p + j) : 0), ys[j]);
+        xs[j] = fma(fb0, (double)(j < A.p ? dir[j] : 0), xs[j]);
+        ys[j] = fma(fb0, (double)(j < A.q ? pick(dir, A.p + j) : 0), ys[j]);
     }
-    T M[D][D];
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -283,10 +278,8 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     int top = 0;
 #pragma unroll
     for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
-    int e2 = 0, nf = 0, ni = 0;
     typename Ar::M mx = 1;  // max of the identity
     bool ok = true;
-    const int steps = S.b[s + 1] - S.b[s];
     for (int st = 0; ok && st < steps; ++st) {
         for (int lev = top; ok && lev >= 1; --lev) {
             for (int ax = A.naxes - 1; ax >= 0; --ax) {
@@ -299,45 +292,66 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
-                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx)) {
-                    S.segV[o] = 2;  // product vanished or overflowed: reference order decides
-                    S.segF[o] = nf;
-                    S.segI[o] = ni;
-                    return;
-                }
+                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                    return 2;  // product vanished or overflowed
                 axis_move(xs, ys, A.p, ax, sign);
             }
         }
     }
+    if (!ok) return 0;
+    if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) return 2;
+    return 1;
+}
+
+// Segment products and their bookkeeping, slot-major: entry (s, slot) at
+// s * ld + slot, matrices at ((s * D + i) * D + j) * ld + slot.
+template <class Ar>
+struct SegStore {
+    typename Ar::T *segM;
+    int32_t *segE, *segV, *segF, *segI;
+    size_t ld;
+};
+
+template <int D, class Ar>
+__device__ void store_segment(const SegStore<Ar> &S, int s, size_t slot, int status,
+                              const typename Ar::T (&M)[D][D], int e2, int nf, int ni) {
+    const size_t o = (size_t)s * S.ld + slot;
+    S.segV[o] = status;
     S.segF[o] = nf;
     S.segI[o] = ni;
-    S.segV[o] = ok ? 1 : 0;
-    if (!ok) return;
-    if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) {
-        S.segV[o] = 2;
-        return;
-    }
+    if (status != 1) return;
     S.segE[o] = e2;
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-        for (int j = 0; j < D; ++j) S.segM[((size_t)((s * D + i) * D + j)) * A.n + t] = M[i][j];
+        for (int j = 0; j < D; ++j) S.segM[((size_t)((s * D + i) * D + j)) * S.ld + slot] = M[i][j];
 }
 
-template <int D, class Ar>
-__global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
-    using T = typename Ar::T;
-    const WalkArgs &A = S.w;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.n) return;
-    // not an integer point, or a segment product that vanished or overflowed:
-    // the reference-order walk decides this trajectory
-    bool exact = false;
-    for (int s = 0; s < S.nseg; ++s) exact = exact || (S.segV[(size_t)s * A.n + t] == 2);
-    if (exact) {
-        walk_exact_traj<D>(A, t);
-        return;
+// Segment boundaries: a fixed table, or a fast plan evaluated in order.
+struct TableBounds {
+    const int *b;
+    __device__ int end(int s) { return b[s + 1]; }
+};
+struct PlanBounds {
+    FastPlan P;
+    int cur;
+    __device__ int end(int) {
+        cur = fast_plan_next(P, cur);
+        return cur;
     }
+};
+
+// Multiplies the nseg segment products of trajectory t (store slot `slot`)
+// in order, takes the snapshots at N-2h, N-h, N (always boundaries) and
+// writes the results.  Returns false, writing nothing, when the reference
+// order must walk t instead (a segment flagged 2, or a product that
+// vanished or overflowed across a boundary).
+template <int D, class Ar, class Bounds>
+__device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S, size_t slot,
+                                 int nseg, Bounds bnd) {
+    using T = typename Ar::T;
+    for (int s = 0; s < nseg; ++s)
+        if (S.segV[(size_t)s * S.ld + slot] == 2) return false;
     const int N = A.N, h = A.h;
     const int cp1 = N - 2 * h, cp2 = N - h;
     T W[D][D];
@@ -347,8 +361,8 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
         for (int j = 0; j < D; ++j) W[i][j] = (i == j) ? Ar::one() : Ar::zero();
     int e2 = 0, nf = 0, ni = 0;
     bool ok = !(N <= 0 || h <= 0 || N - 2 * h <= 0);
-    for (int s = 0; ok && s < S.nseg; ++s) {
-        const size_t o = (size_t)s * A.n + t;
+    for (int s = 0; ok && s < nseg; ++s) {
+        const size_t o = (size_t)s * S.ld + slot;
         nf += S.segF[o];
         ni += S.segI[o];
         if (S.segV[o] != 1) {
@@ -361,7 +375,7 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
         for (int i = 0; i < D; ++i)
 #pragma unroll
             for (int j = 0; j < D; ++j) {
-                G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * A.n + t];
+                G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * S.ld + slot];
                 gmx = fmax(gmx, Ar::mag(G[i][j]));
                 wmx = fmax(wmx, Ar::mag(W[i][j]));
             }
@@ -382,11 +396,9 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
         e2 += S.segE[o];
         // the product of two segments can vanish across their boundary
         typename Ar::M mxo;
-        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, A.collapse, mxo)) {
-            walk_exact_traj<D>(A, t);
-            return;
-        }
-        const int bend = S.b[s + 1];
+        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, A.collapse, mxo))
+            return false;
+        const int bend = bnd.end(s);
         for (int k = 0; k < 3; ++k) {
             const int cp = (k == 0) ? cp1 : (k == 1) ? cp2 : N;
             if (cp != bend) continue;
@@ -408,6 +420,33 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
     if (A.n_fwd) A.n_fwd[t] = nf;
     if (A.n_inv) A.n_inv[t] = ni;
     finish_from_snapshots<D>(A, t, ok);
+    return true;
+}
+
+template <int D, class Ar>
+__global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    const WalkArgs &A = S.w;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (t >= A.n) return;
+    T M[D][D];
+    int e2, nf, ni;
+    const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
+    const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
+    store_segment<D, Ar>(store, s, (size_t)t, st, M, e2, nf, ni);
+}
+
+template <int D, class Ar>
+__global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
+    const WalkArgs &A = S.w;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.n) return;
+    const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
+    // not an integer point, or a product that vanished or overflowed: the
+    // reference-order walk decides this trajectory
+    if (!combine_segments<D, Ar>(A, t, store, (size_t)t, S.nseg, TableBounds{S.b}))
+        walk_exact_traj<D>(A, t);
 }
 
 template <int D, class Ar>

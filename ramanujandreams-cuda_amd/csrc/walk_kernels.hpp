@@ -3,8 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "walk_common.hpp"
+
 namespace cmfw {
-struct WalkArgs;
 constexpr int kMaxRnsModuli = 64;
 constexpr int kMaxRnsSegments = 256;
 
@@ -40,10 +41,22 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
                                uint64_t *crt, int32_t **fallback, hipStream_t st);
-// mixed batches (walk_batch.hip)
-hipError_t launch_walk_batch(int D, const WalkArgs &w, const int32_t *bp, const int32_t *bq,
-                             const double *bz, const int32_t *bN, int h, const int32_t *queue,
-                             int count, int32_t *counter, int32_t *exact, int block, int grid,
+// mixed batches (walk_batch.hip): one rank bucket's launch
+struct BatchLaunch {
+    WalkArgs w;              // shared inputs, snapshots, results
+    const int32_t *bp, *bq;  // per trajectory p, q
+    const double *bz;        // per trajectory z
+    const int32_t *bN;       // per trajectory depth
+    int h;
+    const int32_t *queue;    // this bucket's trajectories, longest walk first
+    int count;
+    int32_t *counter;        // segment-item counter (zeroed before the segment kernel)
+    void *ws;                // batch_bucket_bytes(D, count)
+    int target, min_steps;   // fast plan parameters
+};
+size_t batch_bucket_bytes(int D, int count);
+// part: 1 screen + compaction, 2 segment walk, 4 combine + reference-order walks
+hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, int part,
                              hipStream_t st);
 hipError_t launch_zero_i32(int32_t *p, int n, hipStream_t st);
 // proposal network (mlp.hip)

@@ -48,6 +48,7 @@ def test_batch_matches_reference_order_per_group(engine, orc, tune):
     groups = {}
     for t in range(len(p)):
         groups.setdefault((int(p[t]), int(q[t]), float(z[t]), int(N[t])), []).append(t)
+    err = {"g": [], "r": [], "lg": [], "lr": []}
     for (pp, qq, zz, NN), idx in sorted(groups.items()):
         idx = np.array(idx)
         na = pp + qq
@@ -56,9 +57,62 @@ def test_batch_matches_reference_order_per_group(engine, orc, tune):
         tag = f"{pp}F{qq} z={zz} N={NN}"
         if NN - 2 * h <= 0:
             assert (g["valid"][idx] == 0).all(), tag
+            assert (g["n_fwd"][idx] == 0).all() and (g["n_inv"][idx] == 0).all(), tag
             continue
-        hp = orc.hp_walk_batch(pp, qq, zz, Dg, Sg, NN, h) if NN <= 120 else None
-        check(g[idx], e, tag, hp)
+        # validity, executed steps and limit presence, group by group
+        check(g[idx], e, tag)
+        if NN > 200:
+            continue
+        hp = orc.hp_walk_batch(pp, qq, zz, Dg, Sg, NN, h)
+        v = (e["valid"] == 1) & (hp["ok"] == 1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            eg = _rel(g["p"][idx] / g["q"][idx], hp["ratio"][:, 2])[v]
+            er = _rel(e["p"] / e["q"], hp["ratio"][:, 2])[v]
+        ls = hp["log_scale"][v, 2]
+        err["g"] += list(eg)
+        err["r"] += list(er)
+        err["lg"] += list(np.abs(g["log_scale"][idx][v] - ls) / np.maximum(1.0, np.abs(ls)))
+        err["lr"] += list(np.abs(e["log_scale"][v] - ls) / np.maximum(1.0, np.abs(ls)))
+    # accuracy against quad, pooled over the groups (criteria as for the
+    # fast engine's own test)
+    eg, er = np.array(err["g"]), np.array(err["r"])
+    fin = np.isfinite(eg) & np.isfinite(er)
+    eg, er = eg[fin], er[fin]
+    lg, lr = np.array(err["lg"]), np.array(err["lr"])
+    print(f"  pooled {eg.size} walks vs quad: p/q median/max batch {np.median(eg):.1e}/{eg.max():.1e}"
+          f" reference {np.median(er):.1e}/{er.max():.1e}; log_scale median/max batch "
+          f"{np.median(lg):.1e}/{lg.max():.1e} reference {np.median(lr):.1e}/{lr.max():.1e}",
+          flush=True)
+    assert eg.size >= 25
+    assert np.median(eg) <= 4.0 * np.median(er) + 1e-14
+    assert eg.max() <= 8.0 * er.max() + 1e-12
+    assert np.median(lg) <= 4.0 * np.median(lr) + 1e-12
+    assert lg.max() <= max(8.0 * lr.max(), 1e-5)
+    b.close()
+
+
+def _rel(a, b):
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def test_batch_equals_fast_walk(engine):
+    """A trajectory's batch result is bit-identical to cmfw_walk_f64_fast's:
+    both walk the same depth plan, which depends on N and h only."""
+    p, q, z, N, h = 6, 5, 0.5, 400, 50
+    rng = np.random.default_rng(3)
+    n = 700
+    D = np.zeros((n, 16), np.int32)
+    D[:, :p] = rng.integers(0, 3, (n, p))      # x axes only: mostly valid walks
+    D[D.sum(1) == 0, 0] = 1
+    S = np.zeros((n, 16))
+    S[:, :p + q] = rng.integers(2, 10, (n, p + q))
+    b = engine.Batch([p] * n, [q] * n, [z] * n, [N] * n, h, D, S)
+    g = engine.results_to_numpy(b.run())
+    f = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D[:, :p + q], S[:, :p + q], N, h))
+    assert g["valid"].sum() > n // 2
+    for fld in g.dtype.names:
+        assert np.array_equal(g[fld], f[fld], equal_nan=True), fld
     b.close()
 
 
