@@ -123,6 +123,13 @@ __device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], con
 // exactly as the reference does.  On success `mxo` is the max after
 // rescaling.
 constexpr int kCollapseMaxRank = 4;
+// The reference's unfused order can cancel to an exact 0.0 only while its
+// arithmetic has been exact, i.e. in the first steps from the identity: its
+// per-step 1/max normalisation (cuda/cmf_pfq.cuh:100-123) soon gives every
+// entry a full mantissa, after which a collapse leaves a residue there too.
+// The collapse check is applied to the first kCollapseSteps trajectory steps
+// of a walk.
+constexpr int kCollapseSteps = 8;
 
 template <int D, class Ar>
 __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2, bool force,
@@ -134,11 +141,11 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
 #pragma unroll
         for (int c = 0; c < D; ++c) mx = fmax(mx, Ar::mag(M[r][c]));
     if (!(mx > 0) || !isfinite(mx)) return false;
-    // Only ranks <= 4 are checked: there nilpotent steps make the
-    // reference's exact 0.0 a real outcome (3F2 and 4F3 at z = 1).  At higher
-    // rank a collapse to rounding noise is common (most surviving 8F7 z = 1
-    // zeta walks have one) but the reference carries on with its residue,
-    // as this arithmetic does: validity agrees without a handoff.
+    // Only ranks <= 4 are checked (the caller passes floor = 0 elsewhere):
+    // there nilpotent steps make the reference's exact 0.0 a real outcome
+    // (3F2 and 4F3 at z = 1).  At higher rank a collapse to rounding noise is
+    // common (most surviving 8F7 z = 1 zeta walks have one) but the reference
+    // carries on with its residue, as this arithmetic does.
     if (D <= kCollapseMaxRank && (double)mx < terms * (double)floor) return false;
     const Mg hi = (Mg)ldexp(1.0, Ar::kRangeLog2), lo = (Mg)ldexp(1.0, -Ar::kRangeLog2);
     if (force || mx > hi || mx < lo) {
@@ -292,7 +299,8 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
-                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                const float floor = (b0 + st < kCollapseSteps) ? A.collapse : 0.f;
+                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, floor, mx))
                     return 2;  // product vanished or overflowed
                 axis_move(xs, ys, A.p, ax, sign);
             }
@@ -394,9 +402,11 @@ __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S
             for (int j = 0; j < D; ++j) W[r][j] = row[j];
         }
         e2 += S.segE[o];
-        // the product of two segments can vanish across their boundary
+        // a product that vanished or overflowed across the boundary (the
+        // boundaries lie past the first kCollapseSteps steps: no collapse
+        // check)
         typename Ar::M mxo;
-        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, A.collapse, mxo))
+        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, 0.f, mxo))
             return false;
         const int bend = bnd.end(s);
         for (int k = 0; k < 3; ++k) {
@@ -458,75 +468,6 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
     return hipGetLastError();
-}
-
-// The whole walk of trajectory t in one lane, without depth segments: for
-// batches that fill the GPU by themselves.  The snapshots are taken as the
-// combine kernel takes them (FP64 max-normalisation, log of the scale once
-// per snapshot).  Returns false, writing nothing, when the trajectory must
-// be walked in the reference's order instead (start off the integer grid,
-// or a product that vanished or overflowed).
-template <int D, class Ar>
-__device__ bool walk_full_fast(const WalkArgs &A, int t) {
-    using T = typename Ar::T;
-    int32_t dir[16];
-    double xs[8], ys[8];
-    load_traj(A, t, dir, xs, ys);
-    bool clean = true;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) clean = clean && fast_point(xs[j]) && fast_point(ys[j]);
-    if (!clean) return false;
-    const int N = A.N, h = A.h;
-    const int cp1 = N - 2 * h, cp2 = N - h;
-    bool ok = !(N <= 0 || h <= 0 || N - 2 * h <= 0);
-    T M[D][D];
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-        for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
-    int top = 0;
-#pragma unroll
-    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
-    int e2 = 0, nf = 0, ni = 0;
-    typename Ar::M mx = 1;  // max of the identity
-    for (int step = 1; ok && step <= N; ++step) {
-        for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
-                const int dv = pick(dir, ax);
-                if (abs(dv) < lev) continue;
-                const int sign = dv > 0 ? 1 : -1;
-                bool inverse;
-                double tsc;
-                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse, tsc);
-                nf += inverse ? 0 : 1;
-                ni += inverse ? 1 : 0;
-                if (!ok) break;
-                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
-                    return false;
-                axis_move(xs, ys, A.p, ax, sign);
-            }
-        }
-        if (!ok) break;
-        const int s = (step == cp1) ? 0 : (step == cp2) ? 1 : (step == N) ? 2 : -1;
-        if (s >= 0) {
-            double smx = 0.0;
-#pragma unroll
-            for (int i = 0; i < D; ++i)
-#pragma unroll
-                for (int j = 0; j < D; ++j) smx = fmax(smx, fabs(Ar::to_d(M[i][j])));
-            const double inv = 1.0 / smx;
-#pragma unroll
-            for (int i = 0; i < D; ++i)
-#pragma unroll
-                for (int j = 0; j < D; ++j)
-                    A.snap[((size_t)((s * D + i) * D + j)) * A.sld + t] = Ar::to_d(M[i][j]) * inv;
-            A.snap_ls[(size_t)s * A.sld + t] = (double)e2 * 0.6931471805599453 + log(smx);
-        }
-    }
-    if (A.n_fwd) A.n_fwd[t] = nf;
-    if (A.n_inv) A.n_inv[t] = ni;
-    finish_from_snapshots<D>(A, t, ok);
-    return true;
 }
 
 template <class Ar>
