@@ -41,7 +41,7 @@ struct RnsSegArgs {
     uint8_t crtp[kMaxRnsSegments + 1];  // 1 where b[i] is a CRT point
     RnsConsts C;
     const uint32_t *dtab;   // [np][2*kDenTab+1] Montgomery a^(m-2) of a mod m
-    uint32_t *segP;         // [nseg][np][D][D][n]
+    uint32_t *segP;         // [nseg][D][D][n][np]: the G lanes of a trajectory adjacent
     int32_t *segV;          // [nseg][n]: 1 ok, 0 failed, 2 not an integer point
     int32_t *segF, *segI;   // [nseg][n] executed fwd / inv axis steps
     uint64_t m0inv;         // m0^(m1-2) mod m1
@@ -191,7 +191,8 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j)
-            RA.segP[((((size_t)s * RA.np + k) * D + i) * D + j) * A.n + t] = R[i][j];
+            RA.segP[(((size_t)s * D + i) * D + j) * ((size_t)A.n * RA.np) + (size_t)t * RA.np + k] =
+                R[i][j];
 }
 
 // Garner CRT of one entry on moduli 0 and 1, with the reference's uint64
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
             break;
         }
         // W <- W * P_s (mod m), row by row
-        const uint32_t *P = RA.segP + (((size_t)s * RA.np + (active ? k : 0)) * D * D) * A.n + t;
+        const size_t pst = (size_t)A.n * RA.np;  // stride between matrix entries
+        const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + (size_t)t * RA.np + (active ? k : 0);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             uint32_t row[D];
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
             for (int j = 0; j < D; ++j) {
                 uint64_t acc = 0;
 #pragma unroll
-                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * A.n];
+                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * pst];
                 row[j] = redc(acc, M);
             }
 #pragma unroll
