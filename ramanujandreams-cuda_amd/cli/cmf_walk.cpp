@@ -179,6 +179,8 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     R.res.log_scale = R.alloc<double>(n);
     R.res.valid = R.alloc<int32_t>(n);
     R.res.n_limits = R.alloc<int32_t>(n);
+    R.res.n_fwd = R.alloc<int32_t>(n);
+    R.res.n_inv = R.alloc<int32_t>(n);
     R.ws_bytes = cmfw_workspace_bytes(&desc, n, N, h, n_primes);
     R.ws = R.alloc<char>(R.ws_bytes);
 
@@ -219,6 +221,35 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     HIP_OK(hipMemcpy(lsc.data(), R.res.log_scale, n * 8, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(valid.data(), R.res.valid, n * 4, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(nlim.data(), R.res.n_limits, n * 4, hipMemcpyDeviceToHost));
+    std::vector<int32_t> nfw(n), niv(n);
+    HIP_OK(hipMemcpy(nfw.data(), R.res.n_fwd, n * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(niv.data(), R.res.n_inv, n * 4, hipMemcpyDeviceToHost));
+    // Additional keys (the reference's readers ignore them): executed axis
+    // steps and the algorithmic work they stand for.  f64 / df64: flops of
+    // the reference recurrence per executed step (F_fwd = C + 5d^2 + d + 2,
+    // F_inv = C + 10d^2 + 3d + 2, C = 1.5p(p+1) + 1.5q(q+1) + p + q + 2)
+    // against the FP64 vector peak (78.6 TFLOP/s); rns: modular products of
+    // the reference algorithm per lane (forward 2d^2 + one Fermat power,
+    // inverse d(5d+1) + (d+1) powers, 45 products a power) against 3.3e12/s
+    // (quarter-rate 32-bit multiplies, three a Montgomery product).
+    double sf = 0, si = 0;
+    for (int i = 0; i < n; ++i) {
+        sf += nfw[i];
+        si += niv[i];
+    }
+    const double d = desc.rank, C = 1.5 * p * (p + 1) + 1.5 * q * (q + 1) + p + q + 2;
+    double work, peak;
+    const char *unit;
+    if (is_rns()) {
+        work = n_primes * (sf * (2 * d * d + 45) + si * (d * (5 * d + 1) + (d + 1) * 45));
+        peak = 256 * 2.4e9 * 16 / 3;
+        unit = "modmul";
+    } else {
+        work = sf * (C + 5 * d * d + d + 2) + si * (C + 10 * d * d + 3 * d + 2);
+        peak = 78.6e12;
+        unit = "flop";
+    }
+    const double achieved = (ms > 0) ? work / (ms * 1e-3) : 0.0;
 
     // statistics (cuda/cmf_walk_f64.cu:243-255)
     int n_valid = 0, n_with_limits = 0;
@@ -265,6 +296,10 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
         out << "  \"n_with_limits\": " << n_with_limits << ",\n";
         out << "  \"avg_delta\": "
             << (std::isfinite(avg_delta) ? std::to_string(avg_delta) : "null") << ",\n";
+        out << "  \"n_fwd_steps\": " << jd(sf) << ", \"n_inv_steps\": " << jd(si)
+            << ", \"work_unit\": \"" << unit << "\", \"canonical_work\": " << jd(work)
+            << ", \"achieved_per_s\": " << jd(achieved) << ", \"peak_per_s\": " << jd(peak)
+            << ", \"roofline_frac\": " << jd(achieved / peak) << ",\n";
         out << "  \"results\": [\n";
         const int nr = n < 100 ? n : 100;
         for (int i = 0; i < nr; ++i) {
@@ -293,7 +328,11 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
             << ",\n";
         out << "  \"mem_used_mb\": " << mem_used / 1e6 << ",\n";
         out << "  \"n_valid\": " << n_valid << ", \"n_with_limits\": " << n_with_limits
-            << ", \"avg_delta\": " << jd(avg_delta) << "\n}\n";
+            << ", \"avg_delta\": " << jd(avg_delta) << ",\n";
+        out << "  \"n_fwd_steps\": " << jd(sf) << ", \"n_inv_steps\": " << jd(si)
+            << ", \"work_unit\": \"" << unit << "\", \"canonical_work\": " << jd(work)
+            << ", \"achieved_per_s\": " << jd(achieved) << ", \"peak_per_s\": " << jd(peak)
+            << ", \"roofline_frac\": " << jd(achieved / peak) << "\n}\n";
     }
     out.close();
     if (is_f64()) fprintf(stderr, "  Results written to: %s\n", output_path);
