@@ -144,7 +144,7 @@ class Search:
                             torch.full_like(d, -float("inf")))
         k = top_k if top_k is not None else max(1, int(round(top_frac * self.n)))
         k = min(k, live.numel())
-        top = torch.topk(score, k).indices
+        top = torch.sort(torch.topk(score, k).indices).values
         T.mark("select")
         rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
                           n_primes=self.n_primes)
@@ -160,6 +160,9 @@ class Search:
         s = net.score(X)
         m = min(M, live.numel())
         cand = torch.topk(s, m).indices if m > 0 else live[:0]
+        # back to direction order: the lanes of a wave then share their
+        # axis sequence (no divergence)
+        cand = torch.sort(cand).values
         T.mark("rank")
         res = pr.WALKS[engine](P, Q, Z, dl[:, cand].contiguous(), sl[:, cand].contiguous(),
                                N_DEPTH, H)
@@ -168,7 +171,7 @@ class Search:
         score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
                             torch.full_like(d, -float("inf")))
         v = min(V, m)
-        top = cand[torch.topk(score, v).indices] if v > 0 else cand[:0]
+        top = torch.sort(cand[torch.topk(score, v).indices]).values if v > 0 else cand[:0]
         T.mark("select")
         rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
                           n_primes=self.n_primes)

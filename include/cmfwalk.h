@@ -243,6 +243,10 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       the plan depends on N and h only, so a trajectory's
  *                       result does not depend on the batch it is walked in;
  *   "min_segment_steps" shortest fast f64 / two-float depth segment (8);
+ *   "sort_min"          fast walks of at least this many trajectories are
+ *                       walked in direction order (lanes of a wave share
+ *                       their axis sequence; results stay at the caller's
+ *                       indices) (4096);
  *   "collapse_log2"     fast engines hand a rank <= 4 trajectory to the
  *                       reference order when a product's max falls below
  *                       2^-value of the terms that formed it (exact

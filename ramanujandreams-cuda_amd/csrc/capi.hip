@@ -66,6 +66,9 @@ int g_fast_segments = 64;
 // plan, same results): the segment products need only a chunk's worth of
 // workspace.
 constexpr int kFastChunk = 131072;
+// Fast walks of at least this many trajectories are ordered by direction
+// first (dir_sort.hip); smaller ones are walked in the caller's order.
+int g_sort_min = 4096;
 int g_rns_min_segment_steps = 4;
 // fast engines: collapse floor 2^-g_collapse_log2 (see keep_in_range)
 int g_collapse_log2 = 40;
@@ -217,6 +220,7 @@ size_t cmfw_workspace_bytes(const cmfw_desc *desc, int32_t n_traj, int32_t N, in
     const int nseg = plan_segments(N, h, b);
     size_t need = cmfw::fast_segment_bytes(desc->rank, n_traj < kFastChunk ? n_traj : kFastChunk,
                                            nseg);
+    if (n_traj >= g_sort_min) need += cmfw::dir_sort_bytes(n_traj);
     if (n_primes >= 1 && n_primes <= CMFW_MAX_PRIMES) {
         int rb[cmfw::kMaxRnsSegments + 1];
         uint8_t rc[cmfw::kMaxRnsSegments + 1];
@@ -261,9 +265,11 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
     hipStream_t st = (hipStream_t)stream;
     const int n = in->n_traj;
     const int chunk = n < kFastChunk ? n : kFastChunk;
+    const bool sorted = n >= g_sort_min;
     int b[cmfw::kMaxSegments + 1];
     const int nseg = plan_segments(N, h, b);
-    const size_t seg_need = cmfw::fast_segment_bytes(desc->rank, chunk, nseg);
+    const size_t seg_bytes = cmfw::fast_segment_bytes(desc->rank, chunk, nseg);
+    const size_t seg_need = seg_bytes + (sorted ? cmfw::dir_sort_bytes(n) : 0);
     SnapStore S;
     void *seg = nullptr, *owned = nullptr;
     if (workspace) {
@@ -282,22 +288,31 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
     cmfw::WalkArgs A;
     fill_args(A, desc, in, N, h, out, S);
     hipError_t e = hipSuccess;
+    int32_t *perm = nullptr;
+    if (sorted) e = cmfw::dir_sort(in->dir, in->ld, desc->naxes, n, (char *)seg + seg_bytes, &perm, st);
     for (int t0 = 0; t0 < n && e == hipSuccess; t0 += chunk) {
-        // trajectories [t0, t0 + cnt): the same SoA arrays, offset by t0
         cmfw::WalkArgs C = A;
         C.n = (n - t0 < chunk) ? n - t0 : chunk;
-        C.dir = A.dir + t0;
-        C.start = A.start + t0;
-        C.snap = A.snap + t0;
-        C.snap_ls = A.snap_ls + t0;
-        double **dp[] = {&C.limit, &C.delta, &C.rp, &C.rq, &C.precision, &C.log_scale};
-        for (double **f : dp)
-            if (*f) *f += t0;
-        int32_t **ip[] = {&C.valid, &C.n_limits, &C.n_fwd, &C.n_inv};
-        for (int32_t **f : ip)
-            if (*f) *f += t0;
-        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, st)
-               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, st);
+        const int32_t *cp = nullptr;
+        if (sorted) {
+            // lanes [t0, t0 + cnt) of the direction order; trajectories are
+            // addressed through the permutation
+            cp = perm + t0;
+        } else {
+            // trajectories [t0, t0 + cnt): the same SoA arrays, offset by t0
+            C.dir = A.dir + t0;
+            C.start = A.start + t0;
+            C.snap = A.snap + t0;
+            C.snap_ls = A.snap_ls + t0;
+            double **dp[] = {&C.limit, &C.delta, &C.rp, &C.rq, &C.precision, &C.log_scale};
+            for (double **f : dp)
+                if (*f) *f += t0;
+            int32_t **ip[] = {&C.valid, &C.n_limits, &C.n_fwd, &C.n_inv};
+            for (int32_t **f : ip)
+                if (*f) *f += t0;
+        }
+        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, st)
+               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, st);
     }
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, ff ? "walk_f32x2 launch" : "walk_f64_fast launch");
     if (owned) {
@@ -579,6 +594,11 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "min_segment_steps")) {
         if (value < 1 || value > 1000000) return fail(CMFW_EINVAL, "min_segment_steps out of range");
         g_min_segment_steps = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "sort_min")) {
+        if (value < 0) return fail(CMFW_EINVAL, "sort_min must be >= 0");
+        g_sort_min = value > 0x7fffffff ? 0x7fffffff : (int)value;
         return CMFW_OK;
     }
     if (!strcmp(key, "collapse_log2")) {

@@ -40,6 +40,7 @@ struct SegArgsT {
     int32_t *segE;            // [nseg][n] power-of-two exponent of each segment product
     int32_t *segV;            // [nseg][n] 1 ok, 0 failed, 2 exact path
     int32_t *segF, *segI;     // [nseg][n] executed fwd / inv axis steps
+    const int32_t *perm;      // lane i walks trajectory perm[i] (null: i)
 };
 
 // Integer points below 2^23: exact in FP32 as well as FP64, and so are the
@@ -48,10 +49,69 @@ __device__ __forceinline__ bool fast_point(double v) {
     return v == rint(v) && fabs(v) < 8388608.0;
 }
 
+// Position state of a fast walk, kept small in registers: integer
+// coordinates and the direction packed 8 bits per axis, 4 axes per word.
+struct FastPos {
+    int32_t x[8], y[8];
+    uint32_t dpk[4];
+};
+
+__device__ __forceinline__ int32_t pick8(const int32_t (&a)[8], int idx) {
+    int32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r |= a[j] & -(int32_t)(j == idx);
+    return r;
+}
+
+// direction entry of (wave-uniform) axis ax
+__device__ __forceinline__ int dir_at(const FastPos &P, int ax) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w |= P.dpk[k] & (0u - (uint32_t)(k == (ax >> 2)));
+    return (int)(int8_t)((w >> (8 * (ax & 3))) & 0xffu);
+}
+
+// Position of trajectory t after b0 trajectory steps.  Returns false when
+// t is outside the fast engines' domain: a start off the integer grid or
+// beyond 2^23, a direction entry beyond +-127, or positions that could
+// leave the int32 range within N steps.
+__device__ __forceinline__ bool load_fast(const WalkArgs &A, int t, int b0, FastPos &P, int &top) {
+    bool ok = true;
+    top = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) P.dpk[k] = 0u;
+    int32_t d[16];
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+        d[a] = (a < A.naxes) ? A.dir[(size_t)a * A.ld + t] : 0;
+        ok = ok && d[a] >= -127 && d[a] <= 127;
+        top = max(top, abs(d[a]));
+        P.dpk[a >> 2] |= ((uint32_t)d[a] & 0xffu) << (8 * (a & 3));
+    }
+    const double reach = (double)A.N * (double)top;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double sx = (j < A.p) ? A.start[(size_t)j * A.ld + t] : 0.0;
+        const double sy = (j < A.q) ? A.start[(size_t)(A.p + j) * A.ld + t] : 0.0;
+        ok = ok && fast_point(sx) && fast_point(sy) && fabs(sx) + reach < 1073741824.0 &&
+             fabs(sy) + reach < 1073741824.0;
+        P.x[j] = ok ? (int32_t)sx + b0 * ((j < A.p) ? d[j] : 0) : 0;
+        P.y[j] = ok ? (int32_t)sy + b0 * ((j < A.q) ? d[(A.p + j) & 15] : 0) : 0;
+    }
+    return ok;
+}
+
+// theta-companion column at the evaluation point of an axis step: x_j, and
+// y_j - 1, of the current position, the stepped axis shifted by one on an
+// inverse step (apply_axis_step, cuda/cmf_pfq.cuh:255-291).  The theta
+// polynomials are built with fused multiply-adds from the integer
+// coordinates, the vanishing test of the lead is the reference's
+// (cuda/cmf_pfq.cuh:146-181; the lead is exact for integer points).  `neg`
+// reports a negative evaluation coordinate.
 template <int D, class Ar>
-__device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], const double (&yv)[8],
+__device__ __forceinline__ bool companion_column_fast(const FastPos &P, int ix, int iy, bool inverse,
                                                       int p, int q, double z,
-                                                      typename Ar::T (&col)[D]) {
+                                                      typename Ar::T (&col)[D], bool &neg) {
     using T = typename Ar::T;
     T px[9], py[9];
 #pragma unroll
@@ -61,16 +121,24 @@ __device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], con
     }
     px[0] = Ar::one();
     py[0] = Ar::one();
+    int sabs = 0;
+    neg = false;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         if (j < p) {
-            const T v = Ar::from_d(xv[j]);
+            const int e = P.x[j] - ((inverse && j == ix) ? 1 : 0);
+            sabs += abs(e);
+            neg = neg || e < 0;
+            const T v = Ar::from_d((double)e);
 #pragma unroll
             for (int k = j + 1; k >= 1; --k) px[k] = Ar::fma(px[k], v, px[k - 1]);
             px[0] = Ar::mul(px[0], v);
         }
         if (j < q) {
-            const T v = Ar::from_d(yv[j]);
+            const int e = P.y[j] - 1 + ((inverse && j == iy) ? 1 : 0);
+            sabs += abs(e);
+            neg = neg || e < 0;
+            const T v = Ar::from_d((double)e);
 #pragma unroll
             for (int k = j + 1; k >= 1; --k) py[k] = Ar::fma(py[k], v, py[k - 1]);
             py[0] = Ar::mul(py[0], v);
@@ -89,17 +157,7 @@ __device__ __forceinline__ bool companion_column_fast(const double (&xv)[8], con
     // lead of D(theta) and the reference's vanishing test; for an integer
     // point the lead is exact in either arithmetic
     const double lead = Ar::to_d(dc[D]);
-    double lscale = 1.0;
-    if (D == p && p == q + 1) {
-        lscale = fmax(1.0, fabs(1.0 - z));
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < p) lscale += fabs(xv[i]);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < q) lscale += fabs(yv[i]);
-    }
+    const double lscale = (D == p && p == q + 1) ? fmax(1.0, fabs(1.0 - z)) : 1.0 + (double)sabs;
     if (!isfinite(lead) || fabs(lead) <= 64.0 * kEps * lscale) return false;
     const T nil = Ar::neg(Ar::recip(dc[D]));
     bool ok = true;
@@ -161,19 +219,25 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
     return true;
 }
 
-// One axis step M <- M * (I + C/a) or M * (I + C/a)^-1.  `tsc` receives a
-// bound of the step matrix's entries (max |M| times tsc bounds the terms
-// of the product's entries).
+// One axis step M <- M * (I + C/a) or M * (I + C/a)^-1 at position P
+// (which the caller moves afterwards).  `tsc` receives a bound of the step
+// matrix's entries (max |M| times tsc bounds the terms of the product's
+// entries).
 template <int D, class Ar>
-__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const double (&xs)[8],
-                                               const double (&ys)[8], int p, int q, double z,
-                                               int ax, int sign, bool &inverse, double &tsc) {
+__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const FastPos &P, int p,
+                                               int q, double z, int ax, int sign, bool &inverse,
+                                               double &tsc) {
     using T = typename Ar::T;
-    double xv[8], yv[8], denom;
-    inverse = axis_eval(xs, ys, p, ax, sign, xv, yv, denom);
+    const bool isx = ax < p;
+    const int ix = isx ? ax : -1, iy = isx ? -1 : ax - p;
+    inverse = isx ? (sign < 0) : (sign > 0);
+    const int pos = isx ? pick8(P.x, ix) : pick8(P.y, iy);
+    const int denom_i = isx ? (inverse ? pos - 1 : pos) : (inverse ? pos : pos - 1);
     T col[D];
-    if (!companion_column_fast<D, Ar>(xv, yv, p, q, z, col)) return false;
-    if (denom == 0.0) return false;  // |denom| <= 64 eps for an integer point
+    bool neg;
+    if (!companion_column_fast<D, Ar>(P, ix, iy, inverse, p, q, z, col, neg)) return false;
+    if (denom_i == 0) return false;  // |denom| <= 64 eps for an integer point
+    const double denom = (double)denom_i;
     const T ia = Ar::recip(Ar::from_d(denom));
     double scol = 0.0;
 #pragma unroll
@@ -206,13 +270,16 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
     T sd = Ar::fma(db, ia, Ar::one());
     // The fast sd decides only when it is far from the reference's threshold
     // 128 eps max(1, |db/a|): more than 2^20 times the size of its own terms.
-    bool neg = false;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) neg = neg || (j < p && xv[j] < 0.0) || (j < q && yv[j] < 0.0);
     if (neg || !(fabs(Ar::to_d(sd)) > 0x1p-20 * (dabs * aia + 1.0))) {
         // near the threshold: decide exactly as the reference does, in its
         // FP64 operation order (this file is compiled without implicit
         // contraction, so the products and sums below round separately)
+        double xv[8], yv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            xv[j] = (double)(P.x[j] - ((inverse && j == ix) ? 1 : 0));
+            yv[j] = (double)(P.y[j] - 1 + ((inverse && j == iy) ? 1 : 0));
+        }
         double cref[D];
         if (!companion_column_ref<D>(xv, yv, p, q, z, cref)) return false;
         const double iad = 1.0 / denom;
@@ -263,46 +330,38 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
 template <int D, class Ar>
 __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typename Ar::T (&M)[D][D],
                             int &e2, int &nf, int &ni) {
-    int32_t dir[16];
-    double xs[8], ys[8];
-    load_traj(A, t, dir, xs, ys);
+    FastPos P;
+    int top;
     nf = ni = e2 = 0;
-    bool clean = true;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) clean = clean && fast_point(xs[j]) && fast_point(ys[j]);
-    if (!clean) return 2;
-    // start of this segment: start + b0 * dir (exact integers)
-    const double fb0 = (double)b0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        xs[j] = fma(fb0, (double)(j < A.p ? dir[j] : 0), xs[j]);
-        ys[j] = fma(fb0, (double)(j < A.q ? pick(dir, A.p + j) : 0), ys[j]);
-    }
+    if (!load_fast(A, t, b0, P, top)) return 2;
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
-    int top = 0;
-#pragma unroll
-    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
     typename Ar::M mx = 1;  // max of the identity
     bool ok = true;
     for (int st = 0; ok && st < steps; ++st) {
         for (int lev = top; ok && lev >= 1; --lev) {
             for (int ax = A.naxes - 1; ax >= 0; --ax) {
-                const int dv = pick(dir, ax);
+                const int dv = dir_at(P, ax);
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 bool inverse;
                 double tsc;
-                ok = axis_step_fast<D, Ar>(M, xs, ys, A.p, A.q, A.z, ax, sign, inverse, tsc);
+                ok = axis_step_fast<D, Ar>(M, P, A.p, A.q, A.z, ax, sign, inverse, tsc);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
                 const float floor = (b0 + st < kCollapseSteps) ? A.collapse : 0.f;
                 if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, floor, mx))
                     return 2;  // product vanished or overflowed
-                axis_move(xs, ys, A.p, ax, sign);
+                // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
+                const bool isx = ax < A.p;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    P.x[j] += (isx && j == ax) ? sign : 0;
+                    P.y[j] += (!isx && j == ax - A.p) ? sign : 0;
+                }
             }
         }
     }
@@ -437,25 +496,27 @@ template <int D, class Ar>
 __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     using T = typename Ar::T;
     const WalkArgs &A = S.w;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
-    if (t >= A.n) return;
+    if (i >= A.n) return;
+    const int t = S.perm ? S.perm[i] : i;
     T M[D][D];
     int e2, nf, ni;
     const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
     const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
-    store_segment<D, Ar>(store, s, (size_t)t, st, M, e2, nf, ni);
+    store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
 }
 
 template <int D, class Ar>
 __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
     const WalkArgs &A = S.w;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.n) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int t = S.perm ? S.perm[i] : i;
     const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
     // not an integer point, or a product that vanished or overflowed: the
     // reference-order walk decides this trajectory
-    if (!combine_segments<D, Ar>(A, t, store, (size_t)t, S.nseg, TableBounds{S.b}))
+    if (!combine_segments<D, Ar>(A, t, store, (size_t)i, S.nseg, TableBounds{S.b}))
         walk_exact_traj<D>(A, t);
 }
 
@@ -472,11 +533,12 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
 
 template <class Ar>
 hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
-                            hipStream_t st) {
+                            const int32_t *perm, hipStream_t st) {
     using T = typename Ar::T;
     static_assert(sizeof(T) == 8, "segment storage holds 8-byte values");
     SegArgsT<Ar> S;
     S.w = A;
+    S.perm = perm;
     S.nseg = nseg;
     for (int i = 0; i <= nseg; ++i) S.b[i] = bounds[i];
     char *p = (char *)seg_ws;
