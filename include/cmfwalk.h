@@ -243,6 +243,12 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       the plan depends on N and h only, so a trajectory's
  *                       result does not depend on the batch it is walked in;
  *   "min_segment_steps" shortest fast f64 / two-float depth segment (8);
+ *   "lanes_per_traj"    fast segment kernel: 1 = the whole matrix in one
+ *                       lane, 2 = one matrix row per lane (a lane group per
+ *                       trajectory segment), 0 = rows when a chunk has fewer
+ *                       than "rows_below" (trajectory, segment) pairs (0);
+ *   "rows_below"        see lanes_per_traj (0: the single lane was faster in
+ *                       every measured case);
  *   "sort_min"          fast walks of at least this many trajectories are
  *                       walked in direction order (lanes of a wave share
  *                       their axis sequence; results stay at the caller's

@@ -69,6 +69,15 @@ constexpr int kFastChunk = 131072;
 // Fast walks of at least this many trajectories are ordered by direction
 // first (dir_sort.hip); smaller ones are walked in the caller's order.
 int g_sort_min = 4096;
+// Lanes per (trajectory, segment) of the fast segment kernel: 1 (the whole
+// matrix in one lane), 2 (one row per lane, k_walk_seg_rows), 0 = rows when
+// a chunk has fewer than g_rows_below (trajectory, segment) pairs.  Measured
+// on MI355X the single lane was faster at every batch size tried (400 to
+// 65,536 trajectories, ranks 3-7): the row split issues the wave-uniform
+// column work once per 4-8 trajectories instead of once per 64.  So the
+// automatic choice is the single lane unless rows_below is raised.
+int g_lanes_per_traj = 0;
+int64_t g_rows_below = 0;
 int g_rns_min_segment_steps = 4;
 // fast engines: collapse floor 2^-g_collapse_log2 (see keep_in_range)
 int g_collapse_log2 = 40;
@@ -311,8 +320,10 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
             for (int32_t **f : ip)
                 if (*f) *f += t0;
         }
-        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, st)
-               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, st);
+        const bool rows = g_lanes_per_traj == 2 ||
+                          (g_lanes_per_traj == 0 && (int64_t)C.n * nseg < g_rows_below);
+        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, rows, st)
+               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, rows, st);
     }
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, ff ? "walk_f32x2 launch" : "walk_f64_fast launch");
     if (owned) {
@@ -594,6 +605,16 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "min_segment_steps")) {
         if (value < 1 || value > 1000000) return fail(CMFW_EINVAL, "min_segment_steps out of range");
         g_min_segment_steps = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "lanes_per_traj")) {
+        if (value < 0 || value > 2) return fail(CMFW_EINVAL, "lanes_per_traj must be 0, 1 or 2");
+        g_lanes_per_traj = (int)value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "rows_below")) {
+        if (value < 0) return fail(CMFW_EINVAL, "rows_below must be >= 0");
+        g_rows_below = value;
         return CMFW_OK;
     }
     if (!strcmp(key, "sort_min")) {

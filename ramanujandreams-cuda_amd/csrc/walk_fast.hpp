@@ -193,11 +193,21 @@ template <int D, class Ar>
 __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2, bool force,
                                               double terms, float floor, typename Ar::M &mxo) {
     using Mg = typename Ar::M;
-    Mg mx = 0;
+    // max as a tree (row maxima, then pairs of them): a chain of D^2
+    // dependent max instructions per axis step would stall the wave
+    Mg rm[D];
 #pragma unroll
-    for (int r = 0; r < D; ++r)
+    for (int r = 0; r < D; ++r) {
+        Mg a = Ar::mag(M[r][0]);
 #pragma unroll
-        for (int c = 0; c < D; ++c) mx = fmax(mx, Ar::mag(M[r][c]));
+        for (int c = 1; c < D; ++c) a = fmax(a, Ar::mag(M[r][c]));
+        rm[r] = a;
+    }
+#pragma unroll
+    for (int w = 1; w < D; w *= 2)
+#pragma unroll
+        for (int r = 0; r + w < D; r += 2 * w) rm[r] = fmax(rm[r], rm[r + w]);
+    Mg mx = rm[0];
     if (!(mx > 0) || !isfinite(mx)) return false;
     // Only ranks <= 4 are checked (the caller passes floor = 0 elsewhere):
     // there nilpotent steps make the reference's exact 0.0 a real outcome
@@ -219,55 +229,50 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
     return true;
 }
 
-// One axis step M <- M * (I + C/a) or M * (I + C/a)^-1 at position P
-// (which the caller moves afterwards).  `tsc` receives a bound of the step
-// matrix's entries (max |M| times tsc bounds the terms of the product's
-// entries).
+// The row-independent part of an axis step (apply_axis_step +
+// right_mul_affine_companion, cuda/cmf_pfq.cuh:186-291): the column, 1/a
+// and, for an inverse step, beta and 1/sd.  Returns false where the
+// reference fails (its decisions, see the file comment).  `tsc` receives a
+// bound of the step matrix's entries (max |M| times tsc bounds the terms of
+// the product's entries).
 template <int D, class Ar>
-__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const FastPos &P, int p,
-                                               int q, double z, int ax, int sign, bool &inverse,
-                                               double &tsc) {
+struct StepCoef {
+    typename Ar::T col[D], be[D], ia, isd;
+    bool inverse;
+};
+
+template <int D, class Ar>
+__device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, double z, int ax,
+                                             int sign, StepCoef<D, Ar> &c, double &tsc) {
     using T = typename Ar::T;
     const bool isx = ax < p;
     const int ix = isx ? ax : -1, iy = isx ? -1 : ax - p;
-    inverse = isx ? (sign < 0) : (sign > 0);
+    const bool inverse = isx ? (sign < 0) : (sign > 0);
+    c.inverse = inverse;
     const int pos = isx ? pick8(P.x, ix) : pick8(P.y, iy);
     const int denom_i = isx ? (inverse ? pos - 1 : pos) : (inverse ? pos : pos - 1);
-    T col[D];
     bool neg;
-    if (!companion_column_fast<D, Ar>(P, ix, iy, inverse, p, q, z, col, neg)) return false;
+    if (!companion_column_fast<D, Ar>(P, ix, iy, inverse, p, q, z, c.col, neg)) return false;
     if (denom_i == 0) return false;  // |denom| <= 64 eps for an integer point
     const double denom = (double)denom_i;
-    const T ia = Ar::recip(Ar::from_d(denom));
+    c.ia = Ar::recip(Ar::from_d(denom));
     double scol = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) scol += fabs(Ar::to_d(col[k]));
+    for (int k = 0; k < D; ++k) scol += fabs(Ar::to_d(c.col[k]));
     const double aia = fabs(1.0 / denom);
     tsc = fmax(1.0, aia) * fmax(1.0, scol * aia);
-    if (!inverse) {
+    if (!inverse) return true;
+    c.be[D - 1] = Ar::one();
 #pragma unroll
-        for (int r = 0; r < D; ++r) {
-            T dot = Ar::mul(M[r][0], col[0]);
-#pragma unroll
-            for (int k = 1; k < D; ++k) dot = Ar::fma(M[r][k], col[k], dot);
-#pragma unroll
-            for (int j = 0; j + 1 < D; ++j) M[r][j] = Ar::fma(M[r][j + 1], ia, M[r][j]);
-            M[r][D - 1] = Ar::fma(dot, ia, M[r][D - 1]);
-        }
-        return true;
-    }
-    T be[D];
-    be[D - 1] = Ar::one();
-#pragma unroll
-    for (int j = D - 2; j >= 0; --j) be[j] = Ar::mul(Ar::neg(be[j + 1]), ia);
-    T db = Ar::mul(be[0], col[0]);
-    double dabs = fabs(Ar::to_d(be[0]) * Ar::to_d(col[0]));
+    for (int j = D - 2; j >= 0; --j) c.be[j] = Ar::mul(Ar::neg(c.be[j + 1]), c.ia);
+    T db = Ar::mul(c.be[0], c.col[0]);
+    double dabs = fabs(Ar::to_d(c.be[0]) * Ar::to_d(c.col[0]));
 #pragma unroll
     for (int k = 1; k < D; ++k) {
-        db = Ar::fma(be[k], col[k], db);
-        dabs += fabs(Ar::to_d(be[k]) * Ar::to_d(col[k]));
+        db = Ar::fma(c.be[k], c.col[k], db);
+        dabs += fabs(Ar::to_d(c.be[k]) * Ar::to_d(c.col[k]));
     }
-    T sd = Ar::fma(db, ia, Ar::one());
+    T sd = Ar::fma(db, c.ia, Ar::one());
     // The fast sd decides only when it is far from the reference's threshold
     // 128 eps max(1, |db/a|): more than 2^20 times the size of its own terms.
     if (neg || !(fabs(Ar::to_d(sd)) > 0x1p-20 * (dabs * aia + 1.0))) {
@@ -295,30 +300,55 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
         if (!isfinite(sdr) || fabs(sdr) <= 128.0 * kEps * ssr) return false;
         sd = Ar::from_d(sdr);
 #pragma unroll
-        for (int i = 0; i < D; ++i) col[i] = Ar::from_d(cref[i]);
+        for (int i = 0; i < D; ++i) c.col[i] = Ar::from_d(cref[i]);
     }
     if (!Ar::finite(sd)) return false;
-    const T isd = Ar::recip(sd);
-    {
-        // entries of the inverse step: powers of 1/a up to D-1, over sd
-        double pw = 1.0;
+    c.isd = Ar::recip(sd);
+    // entries of the inverse step: powers of 1/a up to D-1, over sd
+    double pw = 1.0;
 #pragma unroll
-        for (int j = 0; j + 1 < D; ++j) pw *= fmax(1.0, aia);
-        tsc = pw * fmax(1.0, scol) / fmin(1.0, fabs(Ar::to_d(sd)));
+    for (int j = 0; j + 1 < D; ++j) pw *= fmax(1.0, aia);
+    tsc = pw * fmax(1.0, scol) / fmin(1.0, fabs(Ar::to_d(sd)));
+    return true;
+}
+
+// One row of M <- M * (I + C/a) or M * (I + C/a)^-1.
+template <int D, class Ar>
+__device__ __forceinline__ void step_apply_row(typename Ar::T (&m)[D], const StepCoef<D, Ar> &c) {
+    using T = typename Ar::T;
+    if (!c.inverse) {
+        T dot = Ar::mul(m[0], c.col[0]);
+#pragma unroll
+        for (int k = 1; k < D; ++k) dot = Ar::fma(m[k], c.col[k], dot);
+#pragma unroll
+        for (int j = 0; j + 1 < D; ++j) m[j] = Ar::fma(m[j + 1], c.ia, m[j]);
+        m[D - 1] = Ar::fma(dot, c.ia, m[D - 1]);
+        return;
     }
+    T al[D];
+    al[D - 1] = Ar::zero();
+    T da = Ar::zero();
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-        T al[D];
-        al[D - 1] = Ar::zero();
-        T da = Ar::zero();
+    for (int j = D - 2; j >= 0; --j) al[j] = Ar::fma(Ar::neg(al[j + 1]), c.ia, m[j]);
 #pragma unroll
-        for (int j = D - 2; j >= 0; --j) al[j] = Ar::fma(Ar::neg(al[j + 1]), ia, M[r][j]);
+    for (int k = 0; k < D - 1; ++k) da = Ar::fma(al[k], c.col[k], da);
+    const T tt = Ar::mul(Ar::fma(Ar::neg(da), c.ia, m[D - 1]), c.isd);
 #pragma unroll
-        for (int k = 0; k < D - 1; ++k) da = Ar::fma(al[k], col[k], da);
-        const T tt = Ar::mul(Ar::fma(Ar::neg(da), ia, M[r][D - 1]), isd);
+    for (int j = 0; j < D; ++j) m[j] = Ar::fma(c.be[j], tt, al[j]);
+}
+
+// One axis step M <- M * (I + C/a)^(+-1) at position P (which the caller
+// moves afterwards).
+template <int D, class Ar>
+__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const FastPos &P, int p,
+                                               int q, double z, int ax, int sign, bool &inverse,
+                                               double &tsc) {
+    StepCoef<D, Ar> c;
+    const bool ok = step_prepare<D, Ar>(P, p, q, z, ax, sign, c, tsc);
+    inverse = c.inverse;
+    if (!ok) return false;
 #pragma unroll
-        for (int j = 0; j < D; ++j) M[r][j] = Ar::fma(be[j], tt, al[j]);
-    }
+    for (int r = 0; r < D; ++r) step_apply_row<D, Ar>(M[r], c);
     return true;
 }
 
@@ -507,6 +537,111 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
 }
 
+// Row-split segment walk: G lanes per (trajectory, segment), G = D rounded
+// up to a power of two; lane r < D holds row r of the segment product (2D
+// registers instead of 2D^2), lanes r >= D hold none.  The row-independent
+// part of each axis step is the same computation on every lane of the group
+// (one issue for the wave); the range check reduces the matrix max over the
+// group.  Same arithmetic, same result as k_walk_seg, with several times
+// the resident waves: the choice for batches too small to fill the GPU one
+// lane per (trajectory, segment).
+template <int G>
+__device__ __forceinline__ double group_max(double v) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, G));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ float group_max(float v) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, G));
+    return v;
+}
+
+template <int D, int G, class Ar>
+__global__ __launch_bounds__(256) void k_walk_seg_rows(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    using Mg = typename Ar::M;
+    const WalkArgs &A = S.w;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / G, r = gid % G;
+    const int s = blockIdx.y;
+    if (i >= A.n) return;  // whole groups leave together
+    const int t = S.perm ? S.perm[i] : i;
+    const int b0 = S.b[s], steps = S.b[s + 1] - b0;
+    FastPos P;
+    int top, e2 = 0, nf = 0, ni = 0, status = 1;
+    T m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = (j == r) ? Ar::one() : Ar::zero();
+    if (!load_fast(A, t, b0, P, top)) status = 2;
+    const Mg hi = (Mg)ldexp(1.0, Ar::kRangeLog2), lo = (Mg)ldexp(1.0, -Ar::kRangeLog2);
+    Mg mx = 1;  // max of the identity
+    bool ok = status == 1;
+    for (int st = 0; ok && st < steps; ++st) {
+        for (int lev = top; ok && lev >= 1; --lev) {
+            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+                const int dv = dir_at(P, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                StepCoef<D, Ar> c;
+                double tsc;
+                ok = step_prepare<D, Ar>(P, A.p, A.q, A.z, ax, sign, c, tsc);
+                nf += c.inverse ? 0 : 1;
+                ni += c.inverse ? 1 : 0;
+                if (!ok) break;
+                if (r < D) step_apply_row<D, Ar>(m, c);
+                Mg lm = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) lm = fmax(lm, Ar::mag(m[j]));
+                const Mg gm0 = group_max<G>(r < D ? lm : (Mg)0);
+                // keep_in_range on the group's max
+                const float floor = (b0 + st < kCollapseSteps) ? A.collapse : 0.f;
+                if (!(gm0 > 0) || !isfinite(gm0) ||
+                    (D <= kCollapseMaxRank && (double)gm0 < (double)mx * tsc * (double)floor)) {
+                    status = 2;  // product vanished or overflowed
+                    ok = false;
+                    break;
+                }
+                Mg gm = gm0;
+                if (gm > hi || gm < lo) {
+                    const int e = ilogb((double)gm) + 1;
+#pragma unroll
+                    for (int j = 0; j < D; ++j) m[j] = Ar::scale2(m[j], -e);
+                    e2 += e;
+                    gm = (Mg)ldexp((double)gm, -e);
+                }
+                mx = gm;
+                const bool isx = ax < A.p;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    P.x[j] += (isx && j == ax) ? sign : 0;
+                    P.y[j] += (!isx && j == ax - A.p) ? sign : 0;
+                }
+            }
+        }
+    }
+    if (status == 1 && !ok) status = 0;
+    if (status == 1) {
+        // final power-of-two normalisation (keep_in_range with force)
+        const int e = ilogb((double)mx) + 1;
+#pragma unroll
+        for (int j = 0; j < D; ++j) m[j] = Ar::scale2(m[j], -e);
+        e2 += e;
+    }
+    const size_t o = (size_t)s * A.n + i;
+    if (r == 0) {
+        S.segV[o] = status;
+        S.segF[o] = nf;
+        S.segI[o] = ni;
+        if (status == 1) S.segE[o] = e2;
+    }
+    if (status == 1 && r < D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) S.segM[((size_t)((s * D + r) * D + j)) * A.n + i] = m[j];
+    }
+}
+
 template <int D, class Ar>
 __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
     const WalkArgs &A = S.w;
@@ -521,10 +656,16 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
 }
 
 template <int D, class Ar>
-hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
+hipError_t launch_fast_d(const SegArgsT<Ar> &S, bool rows, hipStream_t st) {
     const int bs = 256;
     const int gx = (S.w.n + bs - 1) / bs;
-    hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
+    constexpr int G = D <= 2 ? 2 : D <= 4 ? 4 : 8;
+    if (rows)
+        hipLaunchKernelGGL((k_walk_seg_rows<D, G, Ar>),
+                           dim3((unsigned)(((size_t)S.w.n * G + bs - 1) / bs), S.nseg), dim3(bs), 0,
+                           st, S);
+    else
+        hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
@@ -533,7 +674,7 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, hipStream_t st) {
 
 template <class Ar>
 hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
-                            const int32_t *perm, hipStream_t st) {
+                            const int32_t *perm, bool rows, hipStream_t st) {
     using T = typename Ar::T;
     static_assert(sizeof(T) == 8, "segment storage holds 8-byte values");
     SegArgsT<Ar> S;
@@ -553,13 +694,13 @@ hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nse
     p += align256((size_t)nseg * n * 4);
     S.segI = (int32_t *)p;
     switch (D) {
-        case 2: return launch_fast_d<2, Ar>(S, st);
-        case 3: return launch_fast_d<3, Ar>(S, st);
-        case 4: return launch_fast_d<4, Ar>(S, st);
-        case 5: return launch_fast_d<5, Ar>(S, st);
-        case 6: return launch_fast_d<6, Ar>(S, st);
-        case 7: return launch_fast_d<7, Ar>(S, st);
-        case 8: return launch_fast_d<8, Ar>(S, st);
+        case 2: return launch_fast_d<2, Ar>(S, rows, st);
+        case 3: return launch_fast_d<3, Ar>(S, rows, st);
+        case 4: return launch_fast_d<4, Ar>(S, rows, st);
+        case 5: return launch_fast_d<5, Ar>(S, rows, st);
+        case 6: return launch_fast_d<6, Ar>(S, rows, st);
+        case 7: return launch_fast_d<7, Ar>(S, rows, st);
+        case 8: return launch_fast_d<8, Ar>(S, rows, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -205,3 +205,27 @@ def test_prefix_walk_matches_full_walk_validity(engine, orc):
     first_dies = ymax >= xmax
     assert (pre["valid"][first_dies] == 0).all()
     assert (pre["n_inv"][first_dies] + pre["n_fwd"][first_dies] == 1).all()
+
+
+@pytest.mark.parametrize("walk", ["walk_f64_fast", "walk_f32x2"])
+@pytest.mark.parametrize("p,q,z", [(3, 2, 1.0), (3, 2, 0.5), (4, 3, 0.5), (5, 4, 1.0), (6, 5, 0.5),
+                                   (7, 6, 0.5), (8, 7, 1.0), (8, 7, 0.5)])
+def test_row_split_equals_single_lane(engine, walk, p, q, z):
+    """The row-split segment kernel (one matrix row per lane) performs the
+    single-lane kernel's operations row by row: results are bit-identical."""
+    rng = np.random.default_rng(100 * p + int(4 * z))
+    n = 300
+    D = rng.integers(-2, 4, (n, p + q))
+    D[: n // 2] = np.abs(D[: n // 2])
+    D[D.sum(1) == 0, 0] = 1
+    S = rng.integers(1, 13, (n, p + q)).astype(float)
+    f = getattr(engine, walk)
+    try:
+        engine.set_tuning("lanes_per_traj", 1)
+        a = engine.results_to_numpy(f(p, q, z, D, S, 200, 25))
+        engine.set_tuning("lanes_per_traj", 2)
+        b = engine.results_to_numpy(f(p, q, z, D, S, 200, 25))
+    finally:
+        engine.set_tuning("lanes_per_traj", 0)
+    for fld in a.dtype.names:
+        assert np.array_equal(a[fld], b[fld], equal_nan=True), fld
