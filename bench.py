@@ -13,11 +13,13 @@ they report.
   plain f64   : every survivor walked to N=1000 (fast f64 engine), the top
                 1% of the candidates by delta RNS-verified (all survivors
                 when fewer than 1% survive);
-  plain top-V : the same exhaustive walk, only the top V verified (what the
-                proposal saves beyond reporting fewer candidates);
+  plain top-V : the same exhaustive walk, only its top 100 with ties (at
+                most V) verified (what the proposal saves beyond reporting
+                fewer candidates);
   proposal    : a 100-step walk of every survivor, the proposal network
                 scores its features, the top M are walked to N=1000 and
-                the top V of those by delta RNS-verified.
+                the top 100 of those by delta, with ties at the 100th value
+                (at most V), RNS-verified.
 value = candidates resolved per second by the proposal pipeline whose deep
 walk is f64; the same pipeline with two-float FP32 walks is measured beside
 it, with its recall.  Recall@100: fraction of the exhaustive plain walk's
@@ -107,6 +109,17 @@ class Timer:
         return {k: sum(s[k] for s in out) / n for k in out[0]}
 
 
+def top_with_ties(torch, score, k, cap):
+    """Indices of the k best scores plus every score tied with the k-th,
+    at most cap of them (all when fewer than k)."""
+    n = score.numel()
+    if n <= k:
+        return torch.topk(score, n).indices
+    tau = torch.topk(score, k).values[-1]
+    m = min(cap, int((score >= tau).sum().item()))
+    return torch.topk(score, m).indices
+
+
 class Search:
     def __init__(self, torch, cw, pr, n_cand, n_primes):
         self.torch, self.cw, self.pr = torch, cw, pr
@@ -142,9 +155,11 @@ class Search:
         d = res["delta"]
         score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
                             torch.full_like(d, -float("inf")))
-        k = top_k if top_k is not None else max(1, int(round(top_frac * self.n)))
-        k = min(k, live.numel())
-        top = torch.sort(torch.topk(score, k).indices).values
+        if top_k is not None:
+            top = torch.sort(top_with_ties(torch, score, 100, top_k)).values
+        else:
+            k = min(max(1, int(round(top_frac * self.n))), live.numel())
+            top = torch.sort(torch.topk(score, k).indices).values
         T.mark("select")
         rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
                           n_primes=self.n_primes)
@@ -170,8 +185,8 @@ class Search:
         d = res["delta"]
         score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
                             torch.full_like(d, -float("inf")))
-        v = min(V, m)
-        top = torch.sort(cand[torch.topk(score, v).indices]).values if v > 0 else cand[:0]
+        # the top 100 by delta and their ties, at most V
+        top = torch.sort(cand[top_with_ties(torch, score, 100, V)]).values if m > 0 else cand[:0]
         T.mark("select")
         rns = cw.walk_rns(P, Q, Z, dl[:, top].contiguous(), sl[:, top].contiguous(), N_DEPTH, H,
                           n_primes=self.n_primes)
@@ -428,7 +443,7 @@ def main():
         "config": {"workload": f"8F7 z=1 N=1000 h=50, zeta start, dirs {{0..4}}^15, "
                                f"{args.n_cand} candidates (mt19937_64 seed 42)",
                    "pipeline": f"proposal: screen, 100-step f64 feature walk, MLP rank, top "
-                               f"{args.deep_m} walked f64, top {args.verify_v} RNS-verified "
+                               f"{args.deep_m} walked f64, top 100 + ties (<= {args.verify_v}) RNS-verified "
                                f"({args.n_primes} moduli)",
                    "parallelism": "single-gpu"},
         "speedup_vs_plain_f64": sec_b / sec_p,

@@ -217,7 +217,10 @@ void cmfw_batch_destroy(cmfw_batch *batch);
  * grid are always kept.  Same failure checks as forward_walk's
  * (cuda/cmf_pfq.cuh:348-371) without the matrix products; used to screen
  * large candidate sets before walking them.  keep: device int32 [n];
- * steps (nullable): device int32 [n], axis steps evaluated. */
+ * steps (nullable): device int32 [n], axis steps evaluated.  Without steps
+ * the screen runs in two phases (the exact integer checks first, the FP64
+ * checks then only for the candidates those leave undecided; stream-ordered
+ * memory for a queue of n indices); same keep[]. */
 int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t *keep,
                 int32_t *steps, void *stream);
 
@@ -249,6 +252,12 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       than "rows_below" (trajectory, segment) pairs (0);
  *   "rows_below"        see lanes_per_traj (0: the single lane was faster in
  *                       every measured case);
+ *   "traj_major_below"  single-lane fast walks and RNS walks of fewer
+ *                       trajectories than this put consecutive depth
+ *                       segments of one trajectory in the lanes of a wave
+ *                       (one axis sequence per wave) (4096);
+ *   "screen_two_phase"  cmfw_screen without step counts: 1 (default) the
+ *                       two-phase screen, 0 the one-pass screen (same keep);
  *   "sort_min"          fast walks of at least this many trajectories are
  *                       walked in direction order (lanes of a wave share
  *                       their axis sequence; results stay at the caller's

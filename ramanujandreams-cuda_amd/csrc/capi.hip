@@ -78,6 +78,14 @@ int g_sort_min = 4096;
 // automatic choice is the single lane unless rows_below is raised.
 int g_lanes_per_traj = 0;
 int64_t g_rows_below = 0;
+// Single-lane fast walks of fewer trajectories than this (which are not
+// ordered by direction) put consecutive segments of one trajectory in the
+// lanes of a wave (k_walk_seg_tm), so a wave follows one axis sequence.
+int64_t g_traj_major_below = 4096;
+// Screen without step counts: the two-phase screen (exact integer checks,
+// then the FP64 checks on a queue of the undecided) when 1, the one-pass
+// screen when 0.  Both give the same keep[].
+int g_screen_two_phase = 1;
 int g_rns_min_segment_steps = 4;
 // fast engines: collapse floor 2^-g_collapse_log2 (see keep_in_range)
 int g_collapse_log2 = 40;
@@ -322,8 +330,11 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
         }
         const bool rows = g_lanes_per_traj == 2 ||
                           (g_lanes_per_traj == 0 && (int64_t)C.n * nseg < g_rows_below);
-        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, rows, st)
-               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, rows, st);
+        const int layout = rows ? cmfw::kSegRows
+                           : (n < g_traj_major_below) ? cmfw::kSegTrajMajor
+                                                      : cmfw::kSegSegMajor;
+        e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, layout, st)
+               : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, layout, st);
     }
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, ff ? "walk_f32x2 launch" : "walk_f64_fast launch");
     if (owned) {
@@ -475,7 +486,8 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     fill_args(A, desc, in, N, h, out, S);
     int32_t *fallback = nullptr;
     hipError_t e = cmfw::launch_walk_rns_seg(desc->rank, A, n_primes, rb, rcp, rseg, seg, resid_out,
-                                             crt_out, &fallback, st);
+                                             crt_out, &fallback,
+                                             (int64_t)in->n_traj < g_traj_major_below, st);
     if (e == hipSuccess && rseg > 0)  // start points off the integer grid
         e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, fallback, st);
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_rns launch");
@@ -496,8 +508,19 @@ int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t
     cmfw::WalkArgs A;
     SnapStore S;
     fill_args(A, desc, in, K, 0, nullptr, S);
-    hipError_t e = cmfw::launch_screen(desc->rank, A, K, keep, steps, (hipStream_t)stream);
-    return (e == hipSuccess) ? CMFW_OK : hip_fail(e, "screen launch");
+    hipStream_t st = (hipStream_t)stream;
+    void *ws = nullptr;
+    if (!steps && g_screen_two_phase) {
+        hipError_t e = hipMallocAsync(&ws, cmfw::screen_queue_bytes((int)in->n_traj), st);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(screen queue)");
+    }
+    hipError_t e = cmfw::launch_screen(desc->rank, A, K, keep, steps, ws, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "screen launch");
+    if (ws) {
+        hipError_t e2 = hipFreeAsync(ws, st);
+        if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
+    }
+    return rc;
 }
 
 int64_t cmfw_mlp_param_count(void) { return (int64_t)cmfw::mlp_param_count(); }
@@ -615,6 +638,16 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "rows_below")) {
         if (value < 0) return fail(CMFW_EINVAL, "rows_below must be >= 0");
         g_rows_below = value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "traj_major_below")) {
+        if (value < 0) return fail(CMFW_EINVAL, "traj_major_below must be >= 0");
+        g_traj_major_below = value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "screen_two_phase")) {
+        if (value < 0 || value > 1) return fail(CMFW_EINVAL, "screen_two_phase must be 0 or 1");
+        g_screen_two_phase = (int)value;
         return CMFW_OK;
     }
     if (!strcmp(key, "sort_min")) {

@@ -18,6 +18,10 @@ namespace cmfw {
 
 constexpr double kEps = 2.220446049250313e-16;
 constexpr int kMaxSegments = 80;
+// lane layouts of the fast segment kernel
+constexpr int kSegSegMajor = 0;   // one lane per (trajectory, segment), a wave = 64 trajectories
+constexpr int kSegRows = 1;       // one matrix row per lane (k_walk_seg_rows)
+constexpr int kSegTrajMajor = 2;  // a wave = consecutive segments of one trajectory
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 

@@ -537,6 +537,27 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
 }
 
+// The same segment walk with the lanes of a wave on consecutive segments of
+// one trajectory (item = trajectory * nseg + segment): the lanes share the
+// axis sequence, so a wave executes each axis step once, where lanes of
+// different trajectories would execute the union of their sequences (about
+// twice the steps for directions drawn independently per axis).  For walks
+// too small to be ordered by direction; the segment stores are strided.
+template <int D, class Ar>
+__global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    const WalkArgs &A = S.w;
+    const size_t item = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= (size_t)A.n * S.nseg) return;
+    const int i = (int)(item / S.nseg), s = (int)(item % S.nseg);
+    const int t = S.perm ? S.perm[i] : i;
+    T M[D][D];
+    int e2, nf, ni;
+    const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
+    const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
+    store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
+}
+
 // Row-split segment walk: G lanes per (trajectory, segment), G = D rounded
 // up to a power of two; lane r < D holds row r of the segment product (2D
 // registers instead of 2D^2), lanes r >= D hold none.  The row-independent
@@ -656,11 +677,16 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
 }
 
 template <int D, class Ar>
-hipError_t launch_fast_d(const SegArgsT<Ar> &S, bool rows, hipStream_t st) {
+hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     const int bs = 256;
     const int gx = (S.w.n + bs - 1) / bs;
     constexpr int G = D <= 2 ? 2 : D <= 4 ? 4 : 8;
-    if (rows)
+    const bool rows = layout == kSegRows;
+    if (layout == kSegTrajMajor)
+        hipLaunchKernelGGL((k_walk_seg_tm<D, Ar>),
+                           dim3((unsigned)(((size_t)S.w.n * S.nseg + bs - 1) / bs)), dim3(bs), 0,
+                           st, S);
+    else if (rows)
         hipLaunchKernelGGL((k_walk_seg_rows<D, G, Ar>),
                            dim3((unsigned)(((size_t)S.w.n * G + bs - 1) / bs), S.nseg), dim3(bs), 0,
                            st, S);
@@ -674,7 +700,7 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, bool rows, hipStream_t st) {
 
 template <class Ar>
 hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
-                            const int32_t *perm, bool rows, hipStream_t st) {
+                            const int32_t *perm, int layout, hipStream_t st) {
     using T = typename Ar::T;
     static_assert(sizeof(T) == 8, "segment storage holds 8-byte values");
     SegArgsT<Ar> S;
@@ -694,13 +720,13 @@ hipError_t launch_walk_fast(int D, const WalkArgs &A, const int *bounds, int nse
     p += align256((size_t)nseg * n * 4);
     S.segI = (int32_t *)p;
     switch (D) {
-        case 2: return launch_fast_d<2, Ar>(S, rows, st);
-        case 3: return launch_fast_d<3, Ar>(S, rows, st);
-        case 4: return launch_fast_d<4, Ar>(S, rows, st);
-        case 5: return launch_fast_d<5, Ar>(S, rows, st);
-        case 6: return launch_fast_d<6, Ar>(S, rows, st);
-        case 7: return launch_fast_d<7, Ar>(S, rows, st);
-        case 8: return launch_fast_d<8, Ar>(S, rows, st);
+        case 2: return launch_fast_d<2, Ar>(S, layout, st);
+        case 3: return launch_fast_d<3, Ar>(S, layout, st);
+        case 4: return launch_fast_d<4, Ar>(S, layout, st);
+        case 5: return launch_fast_d<5, Ar>(S, layout, st);
+        case 6: return launch_fast_d<6, Ar>(S, layout, st);
+        case 7: return launch_fast_d<7, Ar>(S, layout, st);
+        case 8: return launch_fast_d<8, Ar>(S, layout, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -26,17 +26,20 @@ inline uint32_t host_modulus(int k) {
     return (k >= 0 && k < kMaxRnsModuli) ? t[k] : 0u;
 }
 hipError_t launch_walk_f64_ref(int D, const WalkArgs &A, hipStream_t st);
-hipError_t launch_screen(int D, const WalkArgs &A, int K, int32_t *keep, int32_t *steps,
+// ws: null, or screen_queue_bytes(n) bytes for the two-phase screen (which
+// writes no step counts: steps must then be null)
+hipError_t launch_screen(int D, const WalkArgs &A, int K, int32_t *keep, int32_t *steps, void *ws,
                          hipStream_t st);
+size_t screen_queue_bytes(int n);
 hipError_t launch_walk_f64_fast(int D, const WalkArgs &A, const int *bounds, int nseg,
-                                void *seg_ws, const int32_t *perm, bool rows, hipStream_t st);
+                                void *seg_ws, const int32_t *perm, int layout, hipStream_t st);
 size_t fast_segment_bytes(int D, int n, int nseg);
 // direction order (dir_sort.hip): perm[i] = the i-th trajectory by direction key
 size_t dir_sort_bytes(int n);
 hipError_t dir_sort(const int32_t *dir, int ld, int naxes, int n, void *ws, int32_t **perm,
                     hipStream_t st);
 hipError_t launch_walk_f32x2(int D, const WalkArgs &A, const int *bounds, int nseg, void *seg_ws,
-                             const int32_t *perm, bool rows, hipStream_t st);
+                             const int32_t *perm, int layout, hipStream_t st);
 hipError_t launch_walk_rns(int D, const WalkArgs &A, int np, uint32_t *resid, uint64_t *crt,
                            const int32_t *only, hipStream_t st);
 size_t rns_seg_bytes(int D, int n, int np, int nseg);
@@ -44,7 +47,8 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
              uint8_t *crt, int maxseg);
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
-                               uint64_t *crt, int32_t **fallback, hipStream_t st);
+                               uint64_t *crt, int32_t **fallback, bool traj_major,
+                               hipStream_t st);
 // mixed batches (walk_batch.hip): one rank bucket's launch
 struct BatchLaunch {
     WalkArgs w;              // shared inputs, snapshots, results

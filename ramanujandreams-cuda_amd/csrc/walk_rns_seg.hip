@@ -48,6 +48,7 @@ struct RnsSegArgs {
     uint32_t *resid;        // [np][D][D][n] or null
     uint64_t *crt;          // [D][D][n] or null
     int32_t *fallback;      // [n]: 1 where the sequential kernel must walk
+    int tm;                 // k_rns_segment lanes: 1 = trajectory-major items (see there)
 };
 
 __global__ void k_rns_dentab(RnsConsts C, int np, uint32_t *dtab) {
@@ -69,9 +70,20 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
     const WalkArgs &A = RA.w;
     const int G = RA.G;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = gtid / G;
     const int k = gtid % G;
-    const int s = blockIdx.y;
+    // seg-major: group = trajectory, blockIdx.y = segment.  trajectory-major
+    // (for small walks): group = trajectory * nseg + segment, so the groups
+    // of a wave walk consecutive segments of one trajectory and share its
+    // axis sequence instead of executing the union of several.
+    int t, s;
+    if (RA.tm) {
+        const int item = gtid / G;
+        t = item / RA.nseg;
+        s = item % RA.nseg;
+    } else {
+        t = gtid / G;
+        s = blockIdx.y;
+    }
     const int lane = threadIdx.x & 63;
     const int base = lane - k;
     const uint64_t group_bits = (G >= 64) ? ~0ull : (((1ull << G) - 1ull) << base);
@@ -338,7 +350,11 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
     const long long threads = (long long)RA.w.n * RA.G;
     const int gx = (int)((threads + bs - 1) / bs);
     if (RA.nseg > 0) {
-        hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
+        if (RA.tm)
+            hipLaunchKernelGGL(k_rns_segment<D>, dim3((int)((threads * RA.nseg + bs - 1) / bs)),
+                               dim3(bs), 0, st, RA);
+        else
+            hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -422,7 +438,8 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
 
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
-                               uint64_t *crt, int32_t **fallback, hipStream_t st) {
+                               uint64_t *crt, int32_t **fallback, bool traj_major,
+                               hipStream_t st) {
     RnsSegArgs RA;
     RA.w = A;
     RA.np = np;
@@ -430,6 +447,8 @@ hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *boun
     while (G < np) G <<= 1;
     RA.G = G;
     RA.nseg = nseg;
+    // trajectory-major item indices must fit the kernel's int arithmetic
+    RA.tm = traj_major && (long long)A.n * nseg * G < (1ll << 30) ? 1 : 0;
     for (int i = 0; i <= nseg; ++i) {
         RA.b[i] = bounds[i];
         RA.crtp[i] = crtp[i];

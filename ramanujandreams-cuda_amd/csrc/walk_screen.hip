@@ -13,6 +13,8 @@
 // decides.  Start points off the integer grid (or beyond 2^23) are always
 // kept.  One lane per candidate, no matrix in registers: high occupancy
 // for the millions of candidates that die within a few axis steps.
+#include <algorithm>
+
 #include "walk_kernels.hpp"
 #include "walk_screen.hpp"
 
@@ -28,24 +30,79 @@ __global__ __launch_bounds__(256) void k_screen(WalkArgs A, int K, int32_t *keep
     if (steps) steps[t] = nf + ni;
 }
 
+// Two-phase screen.  Phase 1 (k_screen_pre): the exact integer checks of
+// screen_int for every candidate, a few integer operations per axis step;
+// the candidates they cannot decide are appended to a queue (one atomic per
+// wave).  Phase 2 (k_screen_queue): screen_traj for the queued candidates
+// only.  Most candidates fail an integer check, so the FP64 column work
+// runs on a small fraction of them, in waves that are all busy.  keep[] is
+// indexed by candidate, so the result does not depend on the queue order
+// and equals k_screen's.
+template <int D>
+__global__ __launch_bounds__(256) void k_screen_pre(WalkArgs A, int K, int32_t *keep,
+                                                    int32_t *queue, int32_t *count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = 0;
+    if (t < A.n) {
+        r = screen_int<D>(A, t, K);
+        keep[t] = (r == 0) ? 0 : 1;  // undecided ones are rewritten by phase 2
+    }
+    const bool need = r == 1;
+    const unsigned long long m = __ballot(need);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (need) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_screen_queue(WalkArgs A, int K, int32_t *keep,
+                                                      const int32_t *queue, const int32_t *count) {
+    const int c = *count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
+        const int t = queue[i];
+        int nf, ni;
+        keep[t] = screen_traj<D>(A, t, K, nf, ni) ? 1 : 0;
+    }
+}
+
+size_t screen_queue_bytes(int n) { return align256((size_t)n * 4) + 256; }
+
 template <int D>
 static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32_t *steps,
-                                  hipStream_t st) {
+                                  void *ws, hipStream_t st) {
     const int bs = 256;
-    hipLaunchKernelGGL(k_screen<D>, dim3((A.n + bs - 1) / bs), dim3(bs), 0, st, A, K, keep, steps);
+    const int g = (A.n + bs - 1) / bs;
+    if (!ws) {
+        hipLaunchKernelGGL(k_screen<D>, dim3(g), dim3(bs), 0, st, A, K, keep, steps);
+        return hipGetLastError();
+    }
+    int32_t *queue = (int32_t *)ws;
+    int32_t *count = (int32_t *)((char *)ws + align256((size_t)A.n * 4));
+    hipError_t e = launch_zero_i32(count, 1, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_screen_pre<D>, dim3(g), dim3(bs), 0, st, A, K, keep, queue, count);
+    // phase 2 grid: enough lanes for a queue of a tenth of the candidates in
+    // one pass, strided beyond that
+    const int g2 = std::max(1, std::min(g, std::max(1024, g / 10)));
+    hipLaunchKernelGGL(k_screen_queue<D>, dim3(g2), dim3(bs), 0, st, A, K, keep,
+                       (const int32_t *)queue, (const int32_t *)count);
     return hipGetLastError();
 }
 
-hipError_t launch_screen(int D, const WalkArgs &A, int K, int32_t *keep, int32_t *steps,
+hipError_t launch_screen(int D, const WalkArgs &A, int K, int32_t *keep, int32_t *steps, void *ws,
                          hipStream_t st) {
     switch (D) {
-        case 2: return launch_screen_d<2>(A, K, keep, steps, st);
-        case 3: return launch_screen_d<3>(A, K, keep, steps, st);
-        case 4: return launch_screen_d<4>(A, K, keep, steps, st);
-        case 5: return launch_screen_d<5>(A, K, keep, steps, st);
-        case 6: return launch_screen_d<6>(A, K, keep, steps, st);
-        case 7: return launch_screen_d<7>(A, K, keep, steps, st);
-        case 8: return launch_screen_d<8>(A, K, keep, steps, st);
+        case 2: return launch_screen_d<2>(A, K, keep, steps, ws, st);
+        case 3: return launch_screen_d<3>(A, K, keep, steps, ws, st);
+        case 4: return launch_screen_d<4>(A, K, keep, steps, ws, st);
+        case 5: return launch_screen_d<5>(A, K, keep, steps, ws, st);
+        case 6: return launch_screen_d<6>(A, K, keep, steps, ws, st);
+        case 7: return launch_screen_d<7>(A, K, keep, steps, ws, st);
+        case 8: return launch_screen_d<8>(A, K, keep, steps, ws, st);
         default: return hipErrorInvalidValue;
     }
 }

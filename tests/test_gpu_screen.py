@@ -48,6 +48,8 @@ def test_screen_candidate_workload(engine):
     pre = engine.walk_prefix_f64(p, q, 1.0, dt, st, 1)
     assert int(keep.sum().item()) == 2771
     assert torch.equal(keep, pre["valid"])
+    # the two-phase screen (no step counts) keeps the same candidates
+    assert torch.equal(engine.screen(p, q, 1.0, dt, st, 1), keep)
     # every candidate evaluates at least its first axis step, survivors all
     # of their |dir|_1 steps
     s = steps.cpu().numpy()
@@ -62,3 +64,30 @@ def test_screen_keeps_off_grid_starts(engine):
     S = np.full((8, p + q), 1.5)
     keep = engine.screen(p, q, 0.5, D, S, 1).cpu().numpy()
     assert np.all(keep == 1)
+
+
+@pytest.mark.parametrize("p,q,z", [(2, 1, 0.5), (3, 2, 0.5), (4, 3, 1.0), (6, 5, 1.0),
+                                   (8, 7, 1.0), (5, 3, 1.0), (3, 3, 0.5)])
+def test_screen_two_phase_equals_one_pass(engine, p, q, z):
+    """The two-phase screen (exact integer lead / denominator checks, then the
+    FP64 checks for the undecided) keeps exactly what the one-pass screen
+    keeps: negative directions, starts at and around zero, a few off the
+    integer grid, K = 1 and 4."""
+    rng = np.random.default_rng(7 * p + q)
+    n = 30000
+    D = rng.integers(-3, 4, (n, p + q))
+    D[np.abs(D).sum(1) == 0, 0] = 1
+    S = rng.integers(-3, 6, (n, p + q)).astype(float)
+    S[: n // 5] = [1.0] * p + [2.0] * q
+    S[n // 5: n // 5 + 50, 0] += 0.5                  # off the grid: kept unchecked
+    print(f"two-phase screen {p}F{q} z={z}", flush=True)
+    try:
+        for K in (1, 4):
+            engine.set_tuning("screen_two_phase", 0)
+            one = engine.screen(p, q, z, D, S, K).cpu().numpy()
+            engine.set_tuning("screen_two_phase", 1)
+            two = engine.screen(p, q, z, D, S, K).cpu().numpy()
+            assert np.array_equal(one, two), (K, int((one != two).sum()))
+            assert 0 < int(two.sum()) < n
+    finally:
+        engine.set_tuning("screen_two_phase", 1)

@@ -229,3 +229,37 @@ def test_row_split_equals_single_lane(engine, walk, p, q, z):
         engine.set_tuning("lanes_per_traj", 0)
     for fld in a.dtype.names:
         assert np.array_equal(a[fld], b[fld], equal_nan=True), fld
+
+
+@pytest.mark.parametrize("walk", ["walk_f64_fast", "walk_f32x2"])
+@pytest.mark.parametrize("p,q,z", [(2, 1, 0.5), (3, 2, 1.0), (4, 3, 0.5), (6, 5, 1.0), (8, 7, 1.0),
+                                   (8, 7, 0.5)])
+def test_traj_major_equals_seg_major(engine, walk, p, q, z):
+    """The trajectory-major segment kernel (the lanes of a wave on consecutive
+    segments of one trajectory) walks each (trajectory, segment) exactly as
+    the segment-major kernel does: bit-identical results, also for a batch
+    that is not a multiple of the wave size and segment counts that split a
+    trajectory across waves."""
+    rng = np.random.default_rng(200 * p + int(4 * z))
+    n = 333
+    D = rng.integers(0, 3, (n, p + q))
+    D[: n // 2, p:] = 0                               # x-only directions walk long
+    D[n // 2: n // 2 + 40] = rng.integers(-2, 3, (40, p + q))
+    D[np.abs(D).sum(1) == 0, 0] = 1
+    S = rng.integers(1, 13, (n, p + q)).astype(float)
+    if z == 1.0:
+        S[n // 3:] = [1.0] * p + [2.0] * q            # zeta starts
+    S[:5, 0] += 0.5                                   # reference-order walks
+    f = getattr(engine, walk)
+    print(f"traj-major {walk} {p}F{q} z={z}", flush=True)
+    try:
+        out = []
+        for below in (0, 1 << 30):
+            engine.set_tuning("traj_major_below", below)
+            out.append(engine.results_to_numpy(f(p, q, z, D, S, 300, 40)))
+    finally:
+        engine.set_tuning("traj_major_below", 4096)
+    a, b = out
+    assert (a["valid"] == 1).sum() > n // 10
+    for fld in a.dtype.names:
+        assert np.array_equal(a[fld], b[fld], equal_nan=True), fld

@@ -96,12 +96,14 @@ def _rns_all(engine, p, q, z, D, S, N, h, np_):
     return engine.results_to_numpy(g), resid.cpu().numpy(), crt.cpu().numpy()
 
 
+@pytest.mark.parametrize("layout", ["traj-major", "seg-major"])
 @pytest.mark.parametrize("p,q,z,N,h", [(8, 7, 1.0, 1000, 50), (4, 3, 0.5, 333, 40),
                                        (6, 5, 0.5, 250, 0), (3, 2, 0.5, 90, 50)])
-def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h):
-    """Depth-segmented RNS (products between CRT points walked in parallel)
-    against the sequential kernel: residues, CRT values and every result
-    field identical."""
+def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h, layout):
+    """Depth-segmented RNS (products between CRT points walked in parallel,
+    the lanes of a wave on one trajectory's segments or on several
+    trajectories) against the sequential kernel: residues, CRT values and
+    every result field identical."""
     rng = np.random.default_rng(N + p)
     n = 300
     D = rng.integers(0, 3, (n, p + q))
@@ -110,13 +112,15 @@ def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h):
     S = rng.integers(1, 9, (n, p + q)).astype(float)
     if z == 1.0:
         S[:] = [1] * p + [2] * q
-    print(f"rns segments vs sequential {p}F{q} N={N} h={h}", flush=True)
-    a, ra, ca = _rns_all(engine, p, q, z, D, S, N, h, 8)
-    engine.set_tuning("rns_segments", 0)
+    print(f"rns segments ({layout}) vs sequential {p}F{q} N={N} h={h}", flush=True)
     try:
+        engine.set_tuning("traj_major_below", 4096 if layout == "traj-major" else 0)
+        a, ra, ca = _rns_all(engine, p, q, z, D, S, N, h, 8)
+        engine.set_tuning("rns_segments", 0)
         b, rb, cb = _rns_all(engine, p, q, z, D, S, N, h, 8)
     finally:
         engine.set_tuning("rns_segments", 1)
+        engine.set_tuning("traj_major_below", 4096)
     assert (a["valid"] == b["valid"]).all()
     v = a["valid"] == 1
     assert np.array_equal(ra[..., v], rb[..., v]) and np.array_equal(ca[..., v], cb[..., v])
