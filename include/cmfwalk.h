@@ -256,6 +256,10 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       trajectories than this put consecutive depth
  *                       segments of one trajectory in the lanes of a wave
  *                       (one axis sequence per wave) (4096);
+ *   "rns_reduce"        1 (default): an RNS walk multiplies the segment
+ *                       products of each CRT interval in parallel before
+ *                       the sequential combine; 0: the combine takes every
+ *                       segment (same residues);
  *   "screen_two_phase"  cmfw_screen without step counts: 1 (default) the
  *                       two-phase screen, 0 the one-pass screen (same keep);
  *   "sort_min"          fast walks of at least this many trajectories are

@@ -18,6 +18,9 @@ struct ArithF64 {
     using M = double;  // magnitude type for max / range checks
     static constexpr const char *name = "f64";
     static constexpr int kRangeLog2 = 400;  // keep max |M| in [2^-400, 2^400]
+    // growth (log2 bound) allowed between two range checks: 2^(400 + 560)
+    // stays below the largest double
+    static constexpr int kGrowBudget = 560;
     __device__ static T zero() { return 0.0; }
     __device__ static T one() { return 1.0; }
     __device__ static T from_d(double x) { return x; }
@@ -41,6 +44,7 @@ struct ArithFF {
     using M = float;
     static constexpr const char *name = "f32x2";
     static constexpr int kRangeLog2 = 60;  // keep max |hi| in [2^-60, 2^60]
+    static constexpr int kGrowBudget = 56;  // 2^(60 + 56) below the largest float
     __device__ static T zero() { return {0.f, 0.f}; }
     __device__ static T one() { return {1.f, 0.f}; }
     __device__ static T from_d(double x) {

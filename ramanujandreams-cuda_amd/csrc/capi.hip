@@ -82,6 +82,9 @@ int64_t g_rows_below = 0;
 // ordered by direction) put consecutive segments of one trajectory in the
 // lanes of a wave (k_walk_seg_tm), so a wave follows one axis sequence.
 int64_t g_traj_major_below = 4096;
+// RNS walks: fold the segment products of each CRT interval in parallel
+// (k_rns_reduce) before the sequential combine.
+int g_rns_reduce = 1;
 // Screen without step counts: the two-phase screen (exact integer checks,
 // then the FP64 checks on a queue of the undecided) when 1, the one-pass
 // screen when 0.  Both give the same keep[].
@@ -487,7 +490,8 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
     int32_t *fallback = nullptr;
     hipError_t e = cmfw::launch_walk_rns_seg(desc->rank, A, n_primes, rb, rcp, rseg, seg, resid_out,
                                              crt_out, &fallback,
-                                             (int64_t)in->n_traj < g_traj_major_below, st);
+                                             (int64_t)in->n_traj < g_traj_major_below,
+                                             g_rns_reduce != 0, st);
     if (e == hipSuccess && rseg > 0)  // start points off the integer grid
         e = cmfw::launch_walk_rns(desc->rank, A, n_primes, resid_out, crt_out, fallback, st);
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_rns launch");
@@ -638,6 +642,11 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     if (!strcmp(key, "rows_below")) {
         if (value < 0) return fail(CMFW_EINVAL, "rows_below must be >= 0");
         g_rows_below = value;
+        return CMFW_OK;
+    }
+    if (!strcmp(key, "rns_reduce")) {
+        if (value < 0 || value > 1) return fail(CMFW_EINVAL, "rns_reduce must be 0 or 1");
+        g_rns_reduce = (int)value;
         return CMFW_OK;
     }
     if (!strcmp(key, "traj_major_below")) {

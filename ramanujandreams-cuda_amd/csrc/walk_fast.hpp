@@ -369,8 +369,18 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
 #pragma unroll
         for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
     typename Ar::M mx = 1;  // max of the identity
+    // Range checks.  Where the collapse check applies (rank <= 4, first
+    // kCollapseSteps steps) the max is taken after every axis step.
+    // Elsewhere only the growth is bounded per axis step (the product's
+    // entries are sums of D terms, each at most max |M| times tsc) and the
+    // max is taken when that bound reaches Ar::kGrowBudget powers of two,
+    // and after every trajectory step (shrinkage, vanished or non-finite
+    // products).  The rescales are exact powers of two, so their timing
+    // does not change the product.
+    int grow = 0;
     bool ok = true;
     for (int st = 0; ok && st < steps; ++st) {
+        const bool watch = D <= kCollapseMaxRank && b0 + st < kCollapseSteps;
         for (int lev = top; ok && lev >= 1; --lev) {
             for (int ax = A.naxes - 1; ax >= 0; --ax) {
                 const int dv = dir_at(P, ax);
@@ -382,9 +392,18 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
-                const float floor = (b0 + st < kCollapseSteps) ? A.collapse : 0.f;
-                if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, floor, mx))
-                    return 2;  // product vanished or overflowed
+                if (watch) {
+                    if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                        return 2;  // product vanished or overflowed
+                } else {
+                    // tsc >= 1; an infinite or NaN bound forces a check
+                    const int g = (tsc < 0x1p1000) ? ilogb(tsc) + 1 : 1000;
+                    grow += g + 3;
+                    if (grow > Ar::kGrowBudget) {
+                        if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) return 2;
+                        grow = 0;
+                    }
+                }
                 // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
                 const bool isx = ax < A.p;
 #pragma unroll
@@ -393,6 +412,10 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
                     P.y[j] += (!isx && j == ax - A.p) ? sign : 0;
                 }
             }
+        }
+        if (ok && !watch && grow > 0) {
+            if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) return 2;
+            grow = 0;
         }
     }
     if (!ok) return 0;

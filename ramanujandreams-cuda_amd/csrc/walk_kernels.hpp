@@ -47,7 +47,7 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
              uint8_t *crt, int maxseg);
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
-                               uint64_t *crt, int32_t **fallback, bool traj_major,
+                               uint64_t *crt, int32_t **fallback, bool traj_major, bool reduce,
                                hipStream_t st);
 // mixed batches (walk_batch.hip): one rank bucket's launch
 struct BatchLaunch {

@@ -49,6 +49,8 @@ struct RnsSegArgs {
     uint64_t *crt;          // [D][D][n] or null
     int32_t *fallback;      // [n]: 1 where the sequential kernel must walk
     int tm;                 // k_rns_segment lanes: 1 = trajectory-major items (see there)
+    int ncrt;               // CRT intervals (boundaries with crtp set)
+    int reduced;            // 1: k_rns_reduce folded each interval into its first segment
 };
 
 __global__ void k_rns_dentab(RnsConsts C, int np, uint32_t *dtab) {
@@ -207,6 +209,91 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
                 R[i][j];
 }
 
+// W <- W * P (mod m) for P at entry stride pst, row by row.
+template <int D>
+__device__ __forceinline__ void mat_mul_mod(uint32_t (&W)[D][D], const uint32_t *P, size_t pst,
+                                            const MontMod &M) {
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        uint32_t row[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            uint64_t acc = 0;
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * pst];
+            row[j] = redc(acc, M);
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) W[r][j] = row[j];
+    }
+}
+
+// Kernel 1b (k_rns_reduce): the segment products of each CRT interval
+// multiplied in order, in parallel over intervals, into the interval's
+// first segment slot (its status and executed-step counts too: the counts
+// up to and including the first segment that failed).  The combine then
+// takes one product per interval: its serial chain is the interval count,
+// not the segment count.  Products mod m are exact, so the result is the
+// same.  One G-lane group per (interval, trajectory), interval-major.
+template <int D>
+__global__ __launch_bounds__(256) void k_rns_reduce(RnsSegArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = gtid % G;
+    const int item = gtid / G;
+    const int c = item / A.n, t = item % A.n;
+    if (c >= RA.ncrt) return;
+    // segments [s0, s1] of interval c
+    int s0 = 0, s1 = RA.nseg - 1, cnt = 0;
+    for (int s = 0; s < RA.nseg; ++s) {
+        if (RA.crtp[s + 1]) {
+            if (cnt == c) {
+                s1 = s;
+                break;
+            }
+            ++cnt;
+            s0 = s + 1;
+        }
+    }
+    if (s1 <= s0) return;
+    const size_t o0 = (size_t)s0 * A.n + t;
+    if (RA.segV[o0] == 2) return;  // off the integer grid: the sequential kernel walks it
+    const bool active = k < RA.np;
+    const MontMod M = RA.C.mods[active ? k : 0];
+    int nf = RA.segF[o0], ni = RA.segI[o0];
+    bool ok = RA.segV[o0] == 1;
+    const size_t pst = (size_t)A.n * RA.np;
+    const size_t col = (size_t)t * RA.np + (active ? k : 0);
+    uint32_t Q[D][D];
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) Q[i][j] = RA.segP[((size_t)s0 * D * D + i * D + j) * pst + col];
+    }
+    for (int s = s0 + 1; ok && s <= s1; ++s) {
+        const size_t o = (size_t)s * A.n + t;
+        nf += RA.segF[o];
+        ni += RA.segI[o];
+        if (RA.segV[o] != 1) {
+            ok = false;
+            break;
+        }
+        mat_mul_mod<D>(Q, RA.segP + ((size_t)s * D * D) * pst + col, pst, M);
+    }
+    if (k == 0) {
+        RA.segV[o0] = ok ? 1 : 0;
+        RA.segF[o0] = nf;
+        RA.segI[o0] = ni;
+    }
+    if (!ok || !active) return;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) RA.segP[((size_t)s0 * D * D + i * D + j) * pst + col] = Q[i][j];
+}
+
 // Garner CRT of one entry on moduli 0 and 1, with the reference's uint64
 // wrap-around (cuda/cmf_walk_rns.cu:232-244).
 __device__ __forceinline__ double crt_garner(uint32_t r0, uint32_t r1, int np, uint64_t m0inv,
@@ -261,28 +348,20 @@ __global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
     int nf = 0, ni = 0;
     double ls = 0.0;
     for (int s = 0; ok && s < RA.nseg; ++s) {
-        const size_t o = (size_t)s * A.n + t;
-        nf += RA.segF[o];
-        ni += RA.segI[o];
-        if (RA.segV[o] != 1) {
-            ok = false;
-            break;
-        }
-        // W <- W * P_s (mod m), row by row
-        const size_t pst = (size_t)A.n * RA.np;  // stride between matrix entries
-        const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + (size_t)t * RA.np + (active ? k : 0);
-#pragma unroll
-        for (int r = 0; r < D; ++r) {
-            uint32_t row[D];
-#pragma unroll
-            for (int j = 0; j < D; ++j) {
-                uint64_t acc = 0;
-#pragma unroll
-                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * pst];
-                row[j] = redc(acc, M);
+        // reduced: an interval's product, status and counts sit in its first
+        // segment's slot
+        if (!RA.reduced || s == 0 || RA.crtp[s]) {
+            const size_t o = (size_t)s * A.n + t;
+            nf += RA.segF[o];
+            ni += RA.segI[o];
+            if (RA.segV[o] != 1) {
+                ok = false;
+                break;
             }
-#pragma unroll
-            for (int j = 0; j < D; ++j) W[r][j] = row[j];
+            // W <- W * P_s (mod m)
+            const size_t pst = (size_t)A.n * RA.np;  // stride between matrix entries
+            mat_mul_mod<D>(W, RA.segP + ((size_t)s * D * D) * pst + (size_t)t * RA.np + (active ? k : 0),
+                           pst, M);
         }
         // a boundary inside a CRT interval only splits the product
         if (!RA.crtp[s + 1]) continue;
@@ -355,6 +434,9 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
                                dim3(bs), 0, st, RA);
         else
             hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
+        if (RA.reduced)
+            hipLaunchKernelGGL(k_rns_reduce<D>, dim3((int)((threads * RA.ncrt + bs - 1) / bs)),
+                               dim3(bs), 0, st, RA);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -438,7 +520,7 @@ int rns_plan(int N, int h, int n, int np, int64_t target_lanes, int min_steps, i
 
 hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *bounds,
                                const uint8_t *crtp, int nseg, void *ws, uint32_t *resid,
-                               uint64_t *crt, int32_t **fallback, bool traj_major,
+                               uint64_t *crt, int32_t **fallback, bool traj_major, bool reduce,
                                hipStream_t st) {
     RnsSegArgs RA;
     RA.w = A;
@@ -449,6 +531,10 @@ hipError_t launch_walk_rns_seg(int D, const WalkArgs &A, int np, const int *boun
     RA.nseg = nseg;
     // trajectory-major item indices must fit the kernel's int arithmetic
     RA.tm = traj_major && (long long)A.n * nseg * G < (1ll << 30) ? 1 : 0;
+    RA.ncrt = 0;
+    for (int i = 1; i <= nseg; ++i) RA.ncrt += crtp[i] ? 1 : 0;
+    // interval reduction when intervals hold several segments
+    RA.reduced = (reduce && RA.ncrt < nseg && (long long)A.n * RA.ncrt * G < (1ll << 30)) ? 1 : 0;
     for (int i = 0; i <= nseg; ++i) {
         RA.b[i] = bounds[i];
         RA.crtp[i] = crtp[i];

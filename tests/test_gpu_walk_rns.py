@@ -96,14 +96,15 @@ def _rns_all(engine, p, q, z, D, S, N, h, np_):
     return engine.results_to_numpy(g), resid.cpu().numpy(), crt.cpu().numpy()
 
 
-@pytest.mark.parametrize("layout", ["traj-major", "seg-major"])
+@pytest.mark.parametrize("layout", ["traj-major", "seg-major", "unreduced"])
 @pytest.mark.parametrize("p,q,z,N,h", [(8, 7, 1.0, 1000, 50), (4, 3, 0.5, 333, 40),
                                        (6, 5, 0.5, 250, 0), (3, 2, 0.5, 90, 50)])
 def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h, layout):
     """Depth-segmented RNS (products between CRT points walked in parallel,
     the lanes of a wave on one trajectory's segments or on several
-    trajectories) against the sequential kernel: residues, CRT values and
-    every result field identical."""
+    trajectories, each CRT interval's products folded in parallel or not)
+    against the sequential kernel: residues, CRT values and every result
+    field identical."""
     rng = np.random.default_rng(N + p)
     n = 300
     D = rng.integers(0, 3, (n, p + q))
@@ -114,13 +115,15 @@ def test_rns_segments_equal_sequential(engine, orc, p, q, z, N, h, layout):
         S[:] = [1] * p + [2] * q
     print(f"rns segments ({layout}) vs sequential {p}F{q} N={N} h={h}", flush=True)
     try:
-        engine.set_tuning("traj_major_below", 4096 if layout == "traj-major" else 0)
+        engine.set_tuning("traj_major_below", 0 if layout == "seg-major" else 4096)
+        engine.set_tuning("rns_reduce", 0 if layout == "unreduced" else 1)
         a, ra, ca = _rns_all(engine, p, q, z, D, S, N, h, 8)
         engine.set_tuning("rns_segments", 0)
         b, rb, cb = _rns_all(engine, p, q, z, D, S, N, h, 8)
     finally:
         engine.set_tuning("rns_segments", 1)
         engine.set_tuning("traj_major_below", 4096)
+        engine.set_tuning("rns_reduce", 1)
     assert (a["valid"] == b["valid"]).all()
     v = a["valid"] == 1
     assert np.array_equal(ra[..., v], rb[..., v]) and np.array_equal(ca[..., v], cb[..., v])
