@@ -242,7 +242,7 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       0 walk every trajectory sequentially;
  *   "target_lanes"      lanes an RNS walk aims to fill (262144):
  *                       trajectories x modulus lanes x segments;
- *   "fast_segments"     depth segments of a fast f64 / two-float walk (64);
+ *   "fast_segments"     depth segments of a fast f64 / two-float walk (128);
  *                       the plan depends on N and h only, so a trajectory's
  *                       result does not depend on the batch it is walked in;
  *   "min_segment_steps" shortest fast f64 / two-float depth segment (8);

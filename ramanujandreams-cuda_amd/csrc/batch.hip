@@ -66,6 +66,7 @@ struct cmfw_batch {
     double *bz = nullptr, *snap = nullptr, *snap_ls = nullptr;
     void *ws[kBuckets] = {nullptr};  // per bucket: segments, survivors, flags
     int target = 64, min_steps = 8;
+    int maxseg[kBuckets] = {0};  // most depth segments of a trajectory in the bucket
     int off[kBuckets + 1] = {0};  // bucket b occupies queue[off[b], off[b+1])
     Shape shape[kBuckets];
     hipStream_t side[kBuckets] = {nullptr};
@@ -105,6 +106,7 @@ hipError_t launch_bucket(cmfw_batch *b, int k, int part, hipStream_t st) {
     L.ws = b->ws[k];
     L.target = b->target;
     L.min_steps = b->min_steps;
+    L.maxseg = b->maxseg[k];
     return cmfw::launch_walk_batch(k + kMinD, L, s.block, grid_of(b, s), part, st);
 }
 
@@ -262,7 +264,15 @@ int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int3
     b->min_steps = cmfw::fast_min_segment_steps();
     for (int k = 0; k < kBuckets && e == hipSuccess; ++k) {
         const int count = b->off[k + 1] - b->off[k];
-        if (count > 0) e = hipMalloc(&b->ws[k], cmfw::batch_bucket_bytes(k + kMinD, count));
+        int ms = 1;
+        for (int i = b->off[k]; i < b->off[k + 1]; ++i) {
+            const int N = spec->N[queue[i]];
+            if (N > 0 && spec->h > 0 && N - 2 * spec->h > 0)
+                ms = std::max(ms, cmfw::fast_plan_count(cmfw::fast_plan(N, spec->h, b->target,
+                                                                        b->min_steps)));
+        }
+        b->maxseg[k] = ms;
+        if (count > 0) e = hipMalloc(&b->ws[k], cmfw::batch_bucket_bytes(k + kMinD, count, ms));
     }
     // snapshots: each bucket indexes [3][D][D][n] with its own D; buckets
     // hold disjoint trajectories, so one buffer sized for D = 8 serves all

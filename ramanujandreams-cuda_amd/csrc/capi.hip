@@ -60,8 +60,12 @@ size_t snap_bytes(int D, int n) {
 // Lanes that keep every SIMD of an MI355X busy: 256 CUs x 4 SIMDs x 64 lanes
 // x 4 waves.  Fewer trajectories than that are split along their depth.
 int64_t g_target_lanes = 262144;
+// Fast-engine depth plan: about 128 segments of at least 8 steps (a walk of
+// N = 1000 in 127 segments of 8 steps).  Small batches are latency-bound:
+// more, shorter segments keep more of the GPU busy; the combine's cost per
+// segment is small next to a segment's walk.
 int g_min_segment_steps = 8;
-int g_fast_segments = 64;
+int g_fast_segments = 128;
 // A fast walk over more trajectories than this runs chunk by chunk (same
 // plan, same results): the segment products need only a chunk's worth of
 // workspace.

@@ -40,6 +40,7 @@ struct BatchArgs {
     int32_t *exact;          // [count] by survivor slot
     SegStore<ArithF64> seg;  // slot = survivor slot, ld = count
     int target, min_steps;   // fast plan parameters
+    int maxseg;              // segment slots per trajectory (the bucket's longest plan)
 };
 
 __device__ __forceinline__ WalkArgs traj_args(const BatchArgs &B, int t) {
@@ -105,7 +106,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_batch_seg(BatchArgs B) {
     const int lane = threadIdx.x & 63;
     const int nk = *B.nkept;
-    const long long total = (long long)kMaxSegments * nk;
+    const long long total = (long long)B.maxseg * nk;
     for (;;) {
         long long base = 0;
         if (lane == 0) base = atomicAdd(B.counter, 64);
@@ -177,10 +178,10 @@ static hipError_t launch_batch_d(const BatchArgs &B, int block, int grid, int pa
     return e;
 }
 
-size_t batch_bucket_bytes(int D, int count) {
-    const size_t n = (size_t)count;
-    return align256((size_t)kMaxSegments * D * D * n * 8) + 4 * align256((size_t)kMaxSegments * n * 4) +
-           4 * align256(n * 4) + align256(4);
+size_t batch_bucket_bytes(int D, int count, int maxseg) {
+    const size_t n = (size_t)count, ms = (size_t)maxseg;
+    return align256(ms * D * D * n * 8) + 4 * align256(ms * n * 4) + 4 * align256(n * 4) +
+           align256(4);
 }
 
 hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, int part,
@@ -197,19 +198,20 @@ hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, i
     B.counter = L.counter;
     B.target = L.target;
     B.min_steps = L.min_steps;
+    B.maxseg = L.maxseg;
     // the bucket's workspace, carved as batch_bucket_bytes lays it out
-    const size_t n = (size_t)L.count;
+    const size_t n = (size_t)L.count, ms = (size_t)L.maxseg;
     char *p = (char *)L.ws;
     B.seg.segM = (double *)p;
-    p += align256((size_t)kMaxSegments * D * D * n * 8);
+    p += align256(ms * D * D * n * 8);
     B.seg.segE = (int32_t *)p;
-    p += align256((size_t)kMaxSegments * n * 4);
+    p += align256(ms * n * 4);
     B.seg.segV = (int32_t *)p;
-    p += align256((size_t)kMaxSegments * n * 4);
+    p += align256(ms * n * 4);
     B.seg.segF = (int32_t *)p;
-    p += align256((size_t)kMaxSegments * n * 4);
+    p += align256(ms * n * 4);
     B.seg.segI = (int32_t *)p;
-    p += align256((size_t)kMaxSegments * n * 4);
+    p += align256(ms * n * 4);
     B.seg.ld = n;
     B.keep = (int32_t *)p;
     p += align256(n * 4);

@@ -17,7 +17,7 @@
 namespace cmfw {
 
 constexpr double kEps = 2.220446049250313e-16;
-constexpr int kMaxSegments = 80;
+constexpr int kMaxSegments = 128;
 // lane layouts of the fast segment kernel
 constexpr int kSegSegMajor = 0;   // one lane per (trajectory, segment), a wave = 64 trajectories
 constexpr int kSegRows = 1;       // one matrix row per lane (k_walk_seg_rows)

@@ -490,15 +490,10 @@ __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S
             break;
         }
         T G[D][D];
-        typename Ar::M gmx = 0, wmx = 0;
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int j = 0; j < D; ++j) {
-                G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * S.ld + slot];
-                gmx = fmax(gmx, Ar::mag(G[i][j]));
-                wmx = fmax(wmx, Ar::mag(W[i][j]));
-            }
+            for (int j = 0; j < D; ++j) G[i][j] = S.segM[((size_t)((s * D + i) * D + j)) * S.ld + slot];
         // W <- W * G, one row at a time
 #pragma unroll
         for (int r = 0; r < D; ++r) {
@@ -516,10 +511,14 @@ __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S
         e2 += S.segE[o];
         // a product that vanished or overflowed across the boundary (the
         // boundaries lie past the first kCollapseSteps steps: no collapse
-        // check)
+        // check).  Segment products end normalised (max in [1/2, 1)), so W
+        // grows by at most D per boundary; in FP64 it is rescaled only when
+        // it leaves the range.  The two-float arithmetic (a float exponent
+        // for both parts) renormalises at every boundary: its low parts
+        // underflow where an entry falls 2^100 below 1.
+        constexpr bool force = Ar::kRangeLog2 < 128;
         typename Ar::M mxo;
-        if (!keep_in_range<D, Ar>(W, e2, true, (double)wmx * (double)gmx * D, 0.f, mxo))
-            return false;
+        if (!keep_in_range<D, Ar>(W, e2, force, 0.0, 0.f, mxo)) return false;
         const int bend = bnd.end(s);
         for (int k = 0; k < 3; ++k) {
             const int cp = (k == 0) ? cp1 : (k == 1) ? cp2 : N;

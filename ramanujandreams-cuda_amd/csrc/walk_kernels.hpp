@@ -59,10 +59,11 @@ struct BatchLaunch {
     const int32_t *queue;    // this bucket's trajectories, longest walk first
     int count;
     int32_t *counter;        // segment-item counter (zeroed before the segment kernel)
-    void *ws;                // batch_bucket_bytes(D, count)
+    void *ws;                // batch_bucket_bytes(D, count, maxseg)
     int target, min_steps;   // fast plan parameters
+    int maxseg;              // most segments of a plan in this bucket
 };
-size_t batch_bucket_bytes(int D, int count);
+size_t batch_bucket_bytes(int D, int count, int maxseg);
 // part: 1 screen + compaction, 2 segment walk, 4 combine + reference-order walks
 hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, int part,
                              hipStream_t st);

@@ -102,9 +102,9 @@ def _with_counts(r, e):
     return out
 
 
-@pytest.mark.parametrize("nseg", [1, 8, 32, 64])
+@pytest.mark.parametrize("nseg", [1, 8, 32, 64, 128])
 def test_fast_segment_plans(engine, orc, nseg):
-    """Different depth-segment plans (one segment up to 64); results agree
+    """Different depth-segment plans (one segment up to 128); results agree
     with the reference order and the quad oracle whatever the plan."""
     p, q, z = 6, 5, 1.0
     n = 300
@@ -119,7 +119,7 @@ def test_fast_segment_plans(engine, orc, nseg):
     try:
         g = engine.results_to_numpy(engine.walk_f64_fast(p, q, z, D, S, N, h))
     finally:
-        engine.set_tuning("fast_segments", 32)
+        engine.set_tuning("fast_segments", 128)
     e = engine.results_to_numpy(engine.walk_f64(p, q, z, D, S, N, h))
     check(g, e, f"nseg={nseg}", orc.hp_walk_batch(p, q, z, D, S, N, h))
 
