@@ -252,10 +252,12 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       than "rows_below" (trajectory, segment) pairs (0);
  *   "rows_below"        see lanes_per_traj (0: the single lane was faster in
  *                       every measured case);
- *   "traj_major_below"  single-lane fast walks and RNS walks of fewer
- *                       trajectories than this put consecutive depth
- *                       segments of one trajectory in the lanes of a wave
- *                       (one axis sequence per wave) (4096);
+ *   "traj_major_below"  single-lane fast walks (of at least 32 depth
+ *                       segments) and RNS walks of fewer trajectories than
+ *                       this put consecutive depth segments of one
+ *                       trajectory in the lanes of a wave (one axis
+ *                       sequence per wave) and combine fast segments with
+ *                       a lane per matrix row (4096);
  *   "rns_reduce"        1 (default): an RNS walk multiplies the segment
  *                       products of each CRT interval in parallel before
  *                       the sequential combine; 0: the combine takes every

@@ -83,8 +83,9 @@ int g_sort_min = 4096;
 int g_lanes_per_traj = 0;
 int64_t g_rows_below = 0;
 // Single-lane fast walks of fewer trajectories than this (which are not
-// ordered by direction) put consecutive segments of one trajectory in the
-// lanes of a wave (k_walk_seg_tm), so a wave follows one axis sequence.
+// ordered by direction) and at least 32 depth segments put consecutive
+// segments of one trajectory in the lanes of a wave (k_walk_seg_tm) and
+// combine them with a lane per matrix row (k_walk_combine_rows).
 int64_t g_traj_major_below = 4096;
 // RNS walks: fold the segment products of each CRT interval in parallel
 // (k_rns_reduce) before the sequential combine.
@@ -337,9 +338,11 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
         }
         const bool rows = g_lanes_per_traj == 2 ||
                           (g_lanes_per_traj == 0 && (int64_t)C.n * nseg < g_rows_below);
+        // trajectory-major pays where a trajectory has many segments (a wave
+        // on one trajectory); with a dozen it splits waves across several
         const int layout = rows ? cmfw::kSegRows
-                           : (n < g_traj_major_below) ? cmfw::kSegTrajMajor
-                                                      : cmfw::kSegSegMajor;
+                           : (n < g_traj_major_below && nseg >= 32) ? cmfw::kSegTrajMajor
+                                                                    : cmfw::kSegSegMajor;
         e = ff ? cmfw::launch_walk_f32x2(desc->rank, C, b, nseg, seg, cp, layout, st)
                : cmfw::launch_walk_f64_fast(desc->rank, C, b, nseg, seg, cp, layout, st);
     }

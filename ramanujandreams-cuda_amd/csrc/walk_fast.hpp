@@ -463,15 +463,13 @@ struct PlanBounds {
 
 // Multiplies the nseg segment products of trajectory t (store slot `slot`)
 // in order, takes the snapshots at N-2h, N-h, N (always boundaries) and
-// writes the results.  Returns false, writing nothing, when the reference
-// order must walk t instead (a segment flagged 2, or a product that
-// vanished or overflowed across a boundary).
+// writes the results.  Returns false, leaving the results unwritten, when
+// the reference order must walk t instead (a segment flagged 2 before any
+// failed one, or a product that vanished or overflowed across a boundary).
 template <int D, class Ar, class Bounds>
 __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S, size_t slot,
                                  int nseg, Bounds bnd) {
     using T = typename Ar::T;
-    for (int s = 0; s < nseg; ++s)
-        if (S.segV[(size_t)s * S.ld + slot] == 2) return false;
     const int N = A.N, h = A.h;
     const int cp1 = N - 2 * h, cp2 = N - h;
     T W[D][D];
@@ -485,6 +483,10 @@ __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S
         const size_t o = (size_t)s * S.ld + slot;
         nf += S.segF[o];
         ni += S.segI[o];
+        // a segment the fast arithmetic cannot decide: the reference order
+        // walks the trajectory (a segment past a failed one is never reached:
+        // the walk fails there in either order)
+        if (S.segV[o] == 2) return false;
         if (S.segV[o] != 1) {
             ok = false;
             break;
@@ -698,6 +700,133 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
         walk_exact_traj<D>(A, t);
 }
 
+// Row-split combine: G lanes per trajectory (G = D rounded up to a power of
+// two), lane r < D holds row r of the running product and multiplies it by
+// each segment product (D^2 fused multiply-adds per boundary instead of
+// D^3 in one lane); the range check and the snapshot's max reduce over the
+// group.  Every entry is computed by the same operations in the same order
+// as in combine_segments, so the results are bit-identical.  For small
+// walks, where one lane per trajectory leaves the combine a long serial
+// chain of dependent loads and products on a handful of waves.
+template <int D, int G, class Ar>
+__global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
+    using T = typename Ar::T;
+    using Mg = typename Ar::M;
+    const WalkArgs &A = S.w;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / G, r = gid % G;
+    if (i >= A.n) return;  // whole groups leave together
+    const int t = S.perm ? S.perm[i] : i;
+    const size_t ld = (size_t)A.n;
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    T w[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) w[j] = (j == r) ? Ar::one() : Ar::zero();
+    int e2 = 0, nf = 0, ni = 0;
+    bool exact = false;
+    bool ok = !(N <= 0 || h <= 0 || N - 2 * h <= 0);
+    const Mg hi = (Mg)ldexp(1.0, Ar::kRangeLog2), lo = (Mg)ldexp(1.0, -Ar::kRangeLog2);
+    constexpr bool force = Ar::kRangeLog2 < 128;  // as in combine_segments
+    // The boundaries form a chain of dependent products; the next segment's
+    // product and flags are loaded while the current one is multiplied, so
+    // each boundary costs one memory latency less.
+    T g[D][D];
+    int sv = 0, sf = 0, si = 0, se = 0;
+    auto load_seg = [&](int s, T (&gg)[D][D], int &v, int &f, int &in, int &e) {
+        const size_t o = (size_t)s * ld + i;
+        v = S.segV[o];
+        f = S.segF[o];
+        in = S.segI[o];
+        e = S.segE[o];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+#pragma unroll
+            for (int j = 0; j < D; ++j) gg[k][j] = S.segM[((size_t)((s * D + k) * D + j)) * ld + i];
+    };
+    if (ok && S.nseg > 0) load_seg(0, g, sv, sf, si, se);
+    for (int s = 0; ok && s < S.nseg; ++s) {
+        T gn[D][D];
+        int vn = 0, fn = 0, inn = 0, en = 0;
+        if (s + 1 < S.nseg) load_seg(s + 1, gn, vn, fn, inn, en);
+        nf += sf;
+        ni += si;
+        if (sv == 2) {  // the reference order decides (as in combine_segments)
+            exact = true;
+            break;
+        }
+        if (sv != 1) {
+            ok = false;
+            break;
+        }
+        // row r of W <- W * G_s
+        T row[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            T acc = Ar::mul(w[0], g[0][j]);
+#pragma unroll
+            for (int k = 1; k < D; ++k) acc = Ar::fma(w[k], g[k][j], acc);
+            row[j] = acc;
+        }
+        if (r < D) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) w[j] = row[j];
+        }
+        e2 += se;
+        // keep_in_range over the group (no collapse check at boundaries)
+        Mg lm = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) lm = fmax(lm, Ar::mag(w[j]));
+        const Mg gm = group_max<G>(r < D ? lm : (Mg)0);
+        if (!(gm > 0) || !isfinite(gm)) {
+            exact = true;  // vanished or overflowed across the boundary
+            break;
+        }
+        if (force || gm > hi || gm < lo) {
+            const int e = ilogb((double)gm) + 1;
+#pragma unroll
+            for (int j = 0; j < D; ++j) w[j] = Ar::scale2(w[j], -e);
+            e2 += e;
+        }
+        const int bend = S.b[s + 1];
+        for (int k = 0; k < 3; ++k) {
+            const int cp = (k == 0) ? cp1 : (k == 1) ? cp2 : N;
+            if (cp != bend) continue;
+            double lmx = 0.0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) lmx = fmax(lmx, fabs(Ar::to_d(w[j])));
+            const double mx = group_max<G>(r < D ? lmx : 0.0);
+            const double inv = 1.0 / mx;
+            if (r < D) {
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    A.snap[((size_t)((k * D + r) * D + j)) * A.sld + t] = Ar::to_d(w[j]) * inv;
+            }
+            if (r == 0) A.snap_ls[(size_t)k * A.sld + t] = (double)e2 * 0.6931471805599453 + log(mx);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+#pragma unroll
+            for (int j = 0; j < D; ++j) g[k][j] = gn[k][j];
+        sv = vn;
+        sf = fn;
+        si = inn;
+        se = en;
+    }
+    if (r != 0) return;
+    // not an integer point, or a product that vanished or overflowed: the
+    // reference-order walk decides this trajectory
+    if (exact) {
+        walk_exact_traj<D>(A, t);
+        return;
+    }
+    if (A.n_fwd) A.n_fwd[t] = nf;
+    if (A.n_inv) A.n_inv[t] = ni;
+    // the group's snapshot rows are read back by lane 0 of the same wave
+    __threadfence_block();
+    finish_from_snapshots<D>(A, t, ok);
+}
+
 template <int D, class Ar>
 hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     const int bs = 256;
@@ -716,7 +845,11 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
         hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
+    if (layout == kSegTrajMajor)
+        hipLaunchKernelGGL((k_walk_combine_rows<D, G, Ar>),
+                           dim3((unsigned)(((size_t)S.w.n * G + bs - 1) / bs)), dim3(bs), 0, st, S);
+    else
+        hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
     return hipGetLastError();
 }
 

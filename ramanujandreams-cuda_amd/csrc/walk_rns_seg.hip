@@ -304,7 +304,15 @@ __device__ __forceinline__ double crt_garner(uint32_t r0, uint32_t r1, int np, u
     }
     const uint64_t m0 = 1073741789ull, m1 = 1073741783ull;
     const uint64_t diff = (r1 >= r0) ? (uint64_t)(r1 - r0) : ((uint64_t)r1 + m1 - r0);
-    const uint64_t tq = (diff * m0inv) % m1;
+    // (diff * m0inv) mod m1 (both < m1 < 2^30): the quotient of the 60-bit
+    // product estimated in FP64 is off by at most one, the remainder is
+    // computed exactly in 64-bit integers and corrected once
+    const uint64_t pr = diff * m0inv;
+    const int64_t qe = (int64_t)floor((double)pr * (1.0 / 1073741783.0));
+    int64_t rr = (int64_t)pr - qe * (int64_t)m1;
+    rr = (rr < 0) ? rr + (int64_t)m1 : rr;
+    rr = (rr >= (int64_t)m1) ? rr - (int64_t)m1 : rr;
+    const uint64_t tq = (uint64_t)rr;
     const uint64_t v = (uint64_t)r0 + m0 * tq;
     const uint64_t half = (m0 * m1) / 2;
     val = v;

@@ -38,13 +38,13 @@ __global__ __launch_bounds__(256) void k_screen(WalkArgs A, int K, int32_t *keep
 // runs on a small fraction of them, in waves that are all busy.  keep[] is
 // indexed by candidate, so the result does not depend on the queue order
 // and equals k_screen's.
-template <int D>
-__global__ __launch_bounds__(256) void k_screen_pre(WalkArgs A, int K, int32_t *keep,
+template <int NAX>
+__global__ __launch_bounds__(256) void k_screen_pre(WalkArgs A, int rank, int K, int32_t *keep,
                                                     int32_t *queue, int32_t *count) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int r = 0;
     if (t < A.n) {
-        r = screen_int<D>(A, t, K);
+        r = screen_int<NAX>(A, rank, t, K);
         keep[t] = (r == 0) ? 0 : 1;  // undecided ones are rewritten by phase 2
     }
     const bool need = r == 1;
@@ -84,7 +84,18 @@ static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32
     int32_t *count = (int32_t *)((char *)ws + align256((size_t)A.n * 4));
     hipError_t e = launch_zero_i32(count, 1, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_screen_pre<D>, dim3(g), dim3(bs), 0, st, A, K, keep, queue, count);
+    switch (A.naxes) {
+#define CMFW_PRE(NAX)                                                                         \
+    case NAX:                                                                                 \
+        hipLaunchKernelGGL(k_screen_pre<NAX>, dim3(g), dim3(bs), 0, st, A, D, K, keep, queue, \
+                           count);                                                            \
+        break;
+        CMFW_PRE(2) CMFW_PRE(3) CMFW_PRE(4) CMFW_PRE(5) CMFW_PRE(6) CMFW_PRE(7) CMFW_PRE(8)
+        CMFW_PRE(9) CMFW_PRE(10) CMFW_PRE(11) CMFW_PRE(12) CMFW_PRE(13) CMFW_PRE(14) CMFW_PRE(15)
+        CMFW_PRE(16)
+#undef CMFW_PRE
+        default: return hipErrorInvalidValue;
+    }
     // phase 2 grid: enough lanes for a queue of a tenth of the candidates in
     // one pass, strided beyond that
     const int g2 = std::max(1, std::min(g, std::max(1024, g / 10)));

@@ -69,45 +69,44 @@ __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
 // lead = sum(y - 1) - sum(x) at the evaluation point), the lead.  For start
 // points on the integer grid below 2^23 both are integers that the
 // reference computes exactly, and it fails its check when one is zero, at
-// that axis step or before it.  Returns 0 (the walk fails in the first K
-// steps), 1 (undecided: screen_traj decides) or 2 (start off the integer
-// grid: kept unchecked, as screen_traj keeps it).
-template <int D>
-__device__ int screen_int(const WalkArgs &A, int t, int K) {
+// that axis step or before it.  NAX = p + q, a compile-time constant so
+// that the axis loop unrolls with constant indices into the direction and
+// position registers.  `rank` is the walk's matrix size.  Returns 0 (the
+// walk fails in the first K steps), 1 (undecided: screen_traj decides) or
+// 2 (start off the integer grid: kept unchecked, as screen_traj keeps it).
+template <int NAX>
+__device__ int screen_int(const WalkArgs &A, int rank, int t, int K) {
     // directions and positions x_0..x_{p-1}, y_0..y_{q-1}, indexed like the
     // axes (the start layout of load_traj)
-    int32_t dir[16], pos[16];
+    int32_t dir[NAX], pos[NAX];
     bool clean = true;
-    int sx = 0, sy = 0;
+    int sx = 0, sy = 0, top = 0;
 #pragma unroll
-    for (int a = 0; a < 16; ++a) {
-        const bool in = a < A.naxes;
-        dir[a] = in ? A.dir[(size_t)a * A.ld + t] : 0;
-        const double v = in ? A.start[(size_t)a * A.ld + t] : 0.0;
+    for (int a = 0; a < NAX; ++a) {
+        dir[a] = A.dir[(size_t)a * A.ld + t];
+        const double v = A.start[(size_t)a * A.ld + t];
         clean = clean && v == rint(v) && fabs(v) < 8388608.0;
         pos[a] = clean ? (int)v : 0;
         if (a < A.p) sx += pos[a];
-        else if (in) sy += pos[a] - 1;
+        else sy += pos[a] - 1;
+        top = max(top, abs(dir[a]));
     }
     if (!clean) return 2;
-    const bool lead_moves = A.z == 1.0 && A.p == A.q + 1 && D == A.q;
-    int top = 0;
-#pragma unroll
-    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    const bool lead_moves = A.z == 1.0 && A.p == A.q + 1 && rank == A.q;
     for (int step = 1; step <= K; ++step) {
         for (int lev = top; lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
-                const int dv = pick(dir, ax);
+#pragma unroll
+            for (int ax = NAX - 1; ax >= 0; --ax) {
+                const int dv = dir[ax];
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 const bool isx = ax < A.p;
                 const bool inverse = isx ? (sign < 0) : (sign > 0);
-                const int v = pick(pos, ax);
+                const int v = pos[ax];
                 const int den = isx ? (inverse ? v - 1 : v) : (inverse ? v : v - 1);
                 const int lead = (sy + ((!isx && inverse) ? 1 : 0)) - (sx - ((isx && inverse) ? 1 : 0));
                 if (den == 0 || (lead_moves && lead == 0)) return 0;
-#pragma unroll
-                for (int a = 0; a < 16; ++a) pos[a] += (a == ax) ? sign : 0;
+                pos[ax] += sign;
                 sx += isx ? sign : 0;
                 sy += isx ? 0 : sign;
             }
