@@ -42,6 +42,7 @@ sys.path.insert(0, PKG)
 
 P, Q, Z, N_DEPTH, H = 8, 7, 1.0, 1000, 50
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak, AMD spec
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # RNS: modular products per second.  MI355X: 256 CUs x 2.4 GHz x 16 lanes
 # per clock of quarter-rate 32-bit multiplies (v_mad_u64_u32 /
 # v_mul_lo_u32), three of them per Montgomery product.
@@ -194,10 +195,12 @@ class Search:
         return {"live": live, "pre": pre, "cand": cand, "res": res, "top": top, "rns": rns}
 
 
-def timed_steps(torch, fn, steps, warmup):
+def timed_steps(torch, fn, steps, warmup, after_warmup=None):
     for _ in range(warmup):
         fn(None)
     torch.cuda.synchronize()
+    if after_warmup:
+        after_warmup()
     T = Timer(torch)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -363,9 +366,17 @@ def main():
 
     # ---- the headline: proposal pipeline, f64 walks
     names_p = ["t0", "screen", "compact", "features", "rank", "deep", "select", "verify"]
+    def events_on():
+        # the library's own HIP events around its main kernels, on the
+        # stream they are launched on, over the timed steps only
+        cw.kernel_time_reset()
+        cw.set_tuning("kernel_events", 1)
+
     out_p, sec_p, T_p = timed_steps(
         torch, lambda T: srch.proposal(T or _NoTimer(), net, "f64", args.deep_m, args.verify_v),
-        args.steps, args.warmup)
+        args.steps, args.warmup, after_warmup=events_on)
+    cw.set_tuning("kernel_events", 0)
+    ktimes = {k: cw.kernel_time(k) for k in ("screen_pre", "walk_seg_f64", "rns_segment")}
     st_p = T_p.stages(names_p)
     log("proposal f64 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_p.items()}))
     # ---- the same pipeline with two-float walks
@@ -395,44 +406,44 @@ def main():
     rec_f = pr.recall_at(100, exhaustive, out_f["top"].cpu().numpy())
     rns_p = cw.results_to_numpy(out_p["rns"])
 
-    # roofline of the headline pipeline's dominant stage
+    # roofline of the headline pipeline's dominant kernel, from the library's
+    # event timings of the timed steps
     fwd_f, inv_f = canonical_flops(7, P, Q)
-    steps = torch.empty(args.n_cand, dtype=torch.int32, device="cuda")
-    cw.screen(P, Q, Z, srch.dir, srch.start, 1, steps=steps)
-    # the screen evaluates the companion column (C + d divisions) and the
-    # step denominator per axis step, no matrix work
-    c_col = canonical_flops(7, P, Q)[0] - 5 * 7 * 7 - 7 - 2 + 7 + 2
-    screen_flops = float(steps.to(torch.float64).sum().item()) * c_col
     deep = cw.results_to_numpy(out_p["res"])
     deep_flops = float(deep["n_fwd"].astype(np.float64).sum() * fwd_f +
                        deep["n_inv"].astype(np.float64).sum() * inv_f)
     feat = cw.results_to_numpy(out_p["pre"])
     feat_flops = float(feat["n_fwd"].astype(np.float64).sum() * fwd_f +
                        feat["n_inv"].astype(np.float64).sum() * inv_f)
-    flops = {"screen": screen_flops, "deep": deep_flops, "features": feat_flops}
-    kern = {"screen": "k_screen<7>", "deep": "k_walk_seg<7,f64> + combine",
-            "features": "k_walk_seg<7,f64> + combine (100 steps)"}
-    dom = max(("screen", "features", "deep", "verify"), key=lambda k: st_p[k])
+    mm = float(rns_modmuls(7, rns_p["n_fwd"].astype(np.float64).sum(),
+                           rns_p["n_inv"].astype(np.float64).sum(), args.n_primes))
     traffic = load_traffic()
-    if dom == "verify":
-        mm = float(rns_modmuls(7, rns_p["n_fwd"].astype(np.float64).sum(),
-                               rns_p["n_inv"].astype(np.float64).sum(), args.n_primes))
-        ach = mm / (st_p[dom] * 1e-3) / 1e12
-        roofline = {"kernel": "k_rns_segment<7> + k_rns_combine<7>", "stage": dom,
-                    "bound": "valu", "achieved": ach, "peak": RNS_PEAK_TMODMUL,
-                    "unit": "Tmodmul/s", "frac": ach / RNS_PEAK_TMODMUL,
-                    "traffic": traffic.get("k_rns_segment<7>"),
-                    "algorithmic_work_per_launch": mm, "launch_ms": st_p[dom]}
-    else:
-        fl = flops[dom]
-        ach = fl / (st_p[dom] * 1e-3) / 1e12
-        roofline = {"kernel": kern[dom], "stage": dom, "bound": "valu", "achieved": ach,
-                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                    "traffic": traffic.get(kern[dom].split(" ")[0]),
-                    "algorithmic_work_per_launch": fl, "launch_ms": st_p[dom]}
-    roofline["deep_f64"] = {"kernel": kern["deep"], "achieved": deep_flops / (st_p["deep"] * 1e-3) / 1e12,
-                            "frac": deep_flops / (st_p["deep"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                            "launch_ms": st_p["deep"]}
+    # per step: algorithmic work, unit, peak, the kernel's name in a trace
+    work = {"rns_segment": (mm, "Tmodmul/s", RNS_PEAK_TMODMUL, "valu", "k_rns_segment<7>"),
+            "walk_seg_f64": (feat_flops + deep_flops, "TFLOP/s", FP64_PEAK_TFLOPS, "valu",
+                             "k_walk_seg<7,f64> / k_walk_seg_tm<7,f64>"),
+            # the integer screen reads each candidate's directions and start
+            "screen_pre": (args.n_cand * (P + Q) * 12.0 / 1e3, "GB/s", HBM_PEAK_GBS, "hbm",
+                           "k_screen_pre<15>")}
+    kernels = {}
+    for k, (ms_tot, launches) in ktimes.items():
+        if launches == 0:
+            continue
+        w, unit, peak, bound, trace_name = work[k]
+        per_launch = w * args.steps / launches
+        avg = ms_tot / launches
+        ach = per_launch / (avg * 1e-3) / 1e12 if unit != "GB/s" else per_launch / (avg * 1e-3) / 1e6
+        kernels[k] = {"kernel": trace_name, "avg_launch_ms": avg, "launches_per_step": launches / args.steps,
+                      "achieved": ach, "unit": unit, "peak": peak, "frac": ach / peak, "bound": bound,
+                      "algorithmic_work_per_launch": per_launch}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_ms"] * kernels[k]["launches_per_step"])
+    d = kernels[dom]
+    roofline = {"kernel": d["kernel"], "bound": d["bound"], "achieved": d["achieved"],
+                "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
+                "traffic": traffic.get(d["kernel"].split(" ")[0]),
+                "algorithmic_work_per_launch": d["algorithmic_work_per_launch"],
+                "launch_ms": d["avg_launch_ms"], "timing": "HIP events around each launch",
+                "kernels": kernels}
 
     value = args.n_cand / sec_p
     line = {

@@ -262,6 +262,8 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       products of each CRT interval in parallel before
  *                       the sequential combine; 0: the combine takes every
  *                       segment (same residues);
+ *   "kernel_events"     1: record kernel timing events (cmfw_kernel_time)
+ *                       (0);
  *   "screen_two_phase"  cmfw_screen without step counts: 1 (default) the
  *                       two-phase screen, 0 the one-pass screen (same keep);
  *   "sort_min"          fast walks of at least this many trajectories are
@@ -276,6 +278,16 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       (4); every CRT point is a boundary.
  * No counterpart in the reference. */
 int cmfw_set_tuning(const char *key, int64_t value);
+
+/* Kernel timing.  With cmfw_set_tuning("kernel_events", 1) the library
+ * records a HIP event pair on the launch's stream around each launch of its
+ * main kernels: "walk_seg_f64", "walk_seg_f32x2" (fast segment walks),
+ * "rns_segment" (RNS segment walks), "screen_pre" (integer screen phase).
+ * cmfw_kernel_time waits for the recorded events of one kernel and returns
+ * the total milliseconds and launch count since the last reset.  No
+ * counterpart in the reference (its runner times whole calls). */
+int cmfw_kernel_time(const char *kernel, double *total_ms, int64_t *launches);
+void cmfw_kernel_time_reset(void);
 
 /* Proposal network ------------------------------------------------------
  * A 64 -> 256 -> 256 -> 256 -> 1 ReLU MLP that scores candidate

@@ -54,6 +54,8 @@ SIGNATURES = {
     "cmfw_workspace_bytes": (C.c_size_t, [C.POINTER(Desc), C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32]),
     "cmfw_set_tuning": (C.c_int, [C.c_char_p, C.c_int64]),
+    "cmfw_kernel_time": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "cmfw_kernel_time_reset": (None, []),
     "cmfw_modulus": (C.c_uint32, [C.c_int32]),
     "cmfw_walk_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                 C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p, C.c_size_t,
@@ -262,6 +264,18 @@ def workspace_bytes(p, q, z, n, N, h, n_primes=8):
 def set_tuning(key, value):
     """Process-wide tuning knob of the library (see include/cmfwalk.h)."""
     _check(load_library().cmfw_set_tuning(key.encode(), int(value)))
+
+
+def kernel_time(kernel):
+    """(total ms, launches) of a library kernel's recorded timing events
+    since the last reset (recorded while set_tuning("kernel_events", 1))."""
+    ms, n = C.c_double(0.0), C.c_int64(0)
+    _check(load_library().cmfw_kernel_time(kernel.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+def kernel_time_reset():
+    load_library().cmfw_kernel_time_reset()
 
 
 def walk_prefix_f64(p, q, z, dirs, starts, K, out=None, stream=None):

@@ -17,6 +17,7 @@ struct ArithF64 {
     using T = double;
     using M = double;  // magnitude type for max / range checks
     static constexpr const char *name = "f64";
+    static constexpr const char *kSegKernel = "walk_seg_f64";  // kernel timing key
     static constexpr int kRangeLog2 = 400;  // keep max |M| in [2^-400, 2^400]
     // growth (log2 bound) allowed between two range checks: 2^(400 + 560)
     // stays below the largest double
@@ -43,6 +44,7 @@ struct ArithFF {
     using T = ff;
     using M = float;
     static constexpr const char *name = "f32x2";
+    static constexpr const char *kSegKernel = "walk_seg_f32x2";  // kernel timing key
     static constexpr int kRangeLog2 = 60;  // keep max |hi| in [2^-60, 2^60]
     static constexpr int kGrowBudget = 56;  // 2^(60 + 56) below the largest float
     __device__ static T zero() { return {0.f, 0.f}; }

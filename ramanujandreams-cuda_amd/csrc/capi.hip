@@ -9,7 +9,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
 #include <random>
+#include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/cmfwalk.h"
 #include "walk_common.hpp"
@@ -94,6 +99,17 @@ int g_rns_reduce = 1;
 // then the FP64 checks on a queue of the undecided) when 1, the one-pass
 // screen when 0.  Both give the same keep[].
 int g_screen_two_phase = 1;
+// Kernel timing: when on, the library records a HIP event pair around each
+// launch of its main kernels, on the launch's stream; cmfw_kernel_time
+// reads them.
+bool g_kernel_events = false;
+std::mutex g_timer_mu;
+struct KernelTimes {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+std::map<std::string, KernelTimes> g_timers;
 int g_rns_min_segment_steps = 4;
 // fast engines: collapse floor 2^-g_collapse_log2 (see keep_in_range)
 int g_collapse_log2 = 40;
@@ -214,6 +230,27 @@ namespace cmfw {
 float collapse_floor() { return ldexpf(1.f, -g_collapse_log2); }
 int fast_segments_target() { return g_fast_segments; }
 int fast_min_segment_steps() { return g_min_segment_steps; }
+hipEvent_t kernel_timer_begin(hipStream_t st) {
+    if (!g_kernel_events) return nullptr;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventRecord(e, st) != hipSuccess) {
+        (void)hipEventDestroy(e);
+        return nullptr;
+    }
+    return e;
+}
+void kernel_timer_end(const char *name, hipEvent_t start, hipStream_t st) {
+    if (!start) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess || hipEventRecord(e, st) != hipSuccess) {
+        if (e) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(start);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    g_timers[name].pending.emplace_back(start, e);
+}
 // error reporting for the other host files of the library
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 int set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
@@ -661,6 +698,11 @@ int cmfw_set_tuning(const char *key, int64_t value) {
         g_traj_major_below = value;
         return CMFW_OK;
     }
+    if (!strcmp(key, "kernel_events")) {
+        if (value < 0 || value > 1) return fail(CMFW_EINVAL, "kernel_events must be 0 or 1");
+        g_kernel_events = value != 0;
+        return CMFW_OK;
+    }
     if (!strcmp(key, "screen_two_phase")) {
         if (value < 0 || value > 1) return fail(CMFW_EINVAL, "screen_two_phase must be 0 or 1");
         g_screen_two_phase = (int)value;
@@ -689,6 +731,40 @@ int cmfw_set_tuning(const char *key, int64_t value) {
         return CMFW_OK;
     }
     return fail(CMFW_EINVAL, "unknown tuning key %s", key);
+}
+
+int cmfw_kernel_time(const char *kernel, double *total_ms, int64_t *launches) {
+    if (!kernel || !total_ms || !launches) return fail(CMFW_EINVAL, "kernel_time: null argument");
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    KernelTimes &k = g_timers[kernel];
+    for (auto &pr : k.pending) {
+        float ms = 0.f;
+        hipError_t e = hipEventSynchronize(pr.second);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, pr.first, pr.second);
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+        if (e != hipSuccess) {
+            k.pending.clear();
+            return hip_fail(e, "kernel_time");
+        }
+        k.ms += ms;
+        ++k.launches;
+    }
+    k.pending.clear();
+    *total_ms = k.ms;
+    *launches = k.launches;
+    return CMFW_OK;
+}
+
+void cmfw_kernel_time_reset(void) {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    for (auto &kv : g_timers)
+        for (auto &pr : kv.second.pending) {
+            (void)hipEventSynchronize(pr.second);
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    g_timers.clear();
 }
 
 }  // extern "C"

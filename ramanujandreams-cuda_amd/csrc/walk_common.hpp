@@ -18,6 +18,9 @@ namespace cmfw {
 
 constexpr double kEps = 2.220446049250313e-16;
 constexpr int kMaxSegments = 128;
+// kernel timing (capi.hip): an event pair around a launch when enabled
+hipEvent_t kernel_timer_begin(hipStream_t st);
+void kernel_timer_end(const char *name, hipEvent_t start, hipStream_t st);
 // lane layouts of the fast segment kernel
 constexpr int kSegSegMajor = 0;   // one lane per (trajectory, segment), a wave = 64 trajectories
 constexpr int kSegRows = 1;       // one matrix row per lane (k_walk_seg_rows)

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
 // axis sequence, so a wave executes each axis step once, where lanes of
 // different trajectories would execute the union of their sequences (about
 // twice the steps for directions drawn independently per axis).  For walks
-// too small to be ordered by direction; the segment stores are strided.
+// too small to be ordered by direction.
 template <int D, class Ar>
 __global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
     using T = typename Ar::T;
@@ -578,8 +578,18 @@ __global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
     T M[D][D];
     int e2, nf, ni;
     const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
-    const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
-    store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
+    // trajectory-contiguous store (item-major: [trajectory][segment][D][D]),
+    // so the lanes of a wave fill whole cache lines; k_walk_combine_rows
+    // reads this layout
+    S.segV[item] = st;
+    S.segF[item] = nf;
+    S.segI[item] = ni;
+    if (st != 1) return;
+    S.segE[item] = e2;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) S.segM[item * (D * D) + a * D + b] = M[a][b];
 }
 
 // Row-split segment walk: G lanes per (trajectory, segment), G = D rounded
@@ -717,7 +727,6 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
     const int i = gid / G, r = gid % G;
     if (i >= A.n) return;  // whole groups leave together
     const int t = S.perm ? S.perm[i] : i;
-    const size_t ld = (size_t)A.n;
     const int N = A.N, h = A.h;
     const int cp1 = N - 2 * h, cp2 = N - h;
     T w[D];
@@ -733,8 +742,9 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
     // each boundary costs one memory latency less.
     T g[D][D];
     int sv = 0, sf = 0, si = 0, se = 0;
+    // k_walk_seg_tm's item-major layout: [trajectory][segment][D][D]
     auto load_seg = [&](int s, T (&gg)[D][D], int &v, int &f, int &in, int &e) {
-        const size_t o = (size_t)s * ld + i;
+        const size_t o = (size_t)i * S.nseg + s;
         v = S.segV[o];
         f = S.segF[o];
         in = S.segI[o];
@@ -742,7 +752,7 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
 #pragma unroll
         for (int k = 0; k < D; ++k)
 #pragma unroll
-            for (int j = 0; j < D; ++j) gg[k][j] = S.segM[((size_t)((s * D + k) * D + j)) * ld + i];
+            for (int j = 0; j < D; ++j) gg[k][j] = S.segM[o * (D * D) + k * D + j];
     };
     if (ok && S.nseg > 0) load_seg(0, g, sv, sf, si, se);
     for (int s = 0; ok && s < S.nseg; ++s) {
@@ -833,6 +843,7 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     const int gx = (S.w.n + bs - 1) / bs;
     constexpr int G = D <= 2 ? 2 : D <= 4 ? 4 : 8;
     const bool rows = layout == kSegRows;
+    hipEvent_t t0 = kernel_timer_begin(st);
     if (layout == kSegTrajMajor)
         hipLaunchKernelGGL((k_walk_seg_tm<D, Ar>),
                            dim3((unsigned)(((size_t)S.w.n * S.nseg + bs - 1) / bs)), dim3(bs), 0,
@@ -843,6 +854,7 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
                            st, S);
     else
         hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
+    kernel_timer_end(Ar::kSegKernel, t0, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (layout == kSegTrajMajor)

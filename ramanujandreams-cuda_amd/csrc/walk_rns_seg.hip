@@ -437,11 +437,13 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
     const long long threads = (long long)RA.w.n * RA.G;
     const int gx = (int)((threads + bs - 1) / bs);
     if (RA.nseg > 0) {
+        hipEvent_t t0 = kernel_timer_begin(st);
         if (RA.tm)
             hipLaunchKernelGGL(k_rns_segment<D>, dim3((int)((threads * RA.nseg + bs - 1) / bs)),
                                dim3(bs), 0, st, RA);
         else
             hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
+        kernel_timer_end("rns_segment", t0, st);
         if (RA.reduced)
             hipLaunchKernelGGL(k_rns_reduce<D>, dim3((int)((threads * RA.ncrt + bs - 1) / bs)),
                                dim3(bs), 0, st, RA);

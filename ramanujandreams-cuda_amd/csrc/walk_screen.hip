@@ -84,6 +84,7 @@ static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32
     int32_t *count = (int32_t *)((char *)ws + align256((size_t)A.n * 4));
     hipError_t e = launch_zero_i32(count, 1, st);
     if (e != hipSuccess) return e;
+    hipEvent_t t0 = kernel_timer_begin(st);
     switch (A.naxes) {
 #define CMFW_PRE(NAX)                                                                         \
     case NAX:                                                                                 \
@@ -96,6 +97,7 @@ static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32
 #undef CMFW_PRE
         default: return hipErrorInvalidValue;
     }
+    kernel_timer_end("screen_pre", t0, st);
     // phase 2 grid: enough lanes for a queue of a tenth of the candidates in
     // one pass, strided beyond that
     const int g2 = std::max(1, std::min(g, std::max(1024, g / 10)));
