@@ -33,14 +33,18 @@ __global__ __launch_bounds__(256) void k_screen(WalkArgs A, int K, int32_t *keep
 // Two-phase screen.  Phase 1 (k_screen_pre): the exact integer checks of
 // screen_int for every candidate, a few integer operations per axis step;
 // the candidates they cannot decide are appended to a queue (one atomic per
-// wave).  Phase 2 (k_screen_queue): screen_traj for the queued candidates
+// block).  Phase 2 (k_screen_queue): screen_traj for the queued candidates
 // only.  Most candidates fail an integer check, so the FP64 column work
 // runs on a small fraction of them, in waves that are all busy.  keep[] is
 // indexed by candidate, so the result does not depend on the queue order
 // and equals k_screen's.
+// One atomic per block of 1024 (the waves' counts are summed in LDS):
+// tens of thousands of same-address atomics, one per wave, serialise in the
+// L2 and cost more than the integer checks themselves.
 template <int NAX>
-__global__ __launch_bounds__(256) void k_screen_pre(WalkArgs A, int rank, int K, int32_t *keep,
-                                                    int32_t *queue, int32_t *count) {
+__global__ __launch_bounds__(1024) void k_screen_pre(WalkArgs A, int rank, int K, int32_t *keep,
+                                                     int32_t *queue, int32_t *count) {
+    __shared__ int wcount[16], wbase[16];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int r = 0;
     if (t < A.n) {
@@ -49,13 +53,21 @@ __global__ __launch_bounds__(256) void k_screen_pre(WalkArgs A, int rank, int K,
     }
     const bool need = r == 1;
     const unsigned long long m = __ballot(need);
-    if (m == 0ull) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (need) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wcount[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        int tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            wbase[w] = tot;
+            tot += wcount[w];
+        }
+        const int base = tot > 0 ? atomicAdd(count, tot) : 0;
+        for (int w = 0; w < nw; ++w) wbase[w] += base;
+    }
+    __syncthreads();
+    if (need) queue[wbase[wave] + __popcll(m & ((1ull << lane) - 1ull))] = t;
 }
 
 template <int D>
@@ -85,11 +97,12 @@ static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32
     hipError_t e = launch_zero_i32(count, 1, st);
     if (e != hipSuccess) return e;
     hipEvent_t t0 = kernel_timer_begin(st);
+    const int gp = (A.n + 1023) / 1024;
     switch (A.naxes) {
-#define CMFW_PRE(NAX)                                                                         \
-    case NAX:                                                                                 \
-        hipLaunchKernelGGL(k_screen_pre<NAX>, dim3(g), dim3(bs), 0, st, A, D, K, keep, queue, \
-                           count);                                                            \
+#define CMFW_PRE(NAX)                                                                            \
+    case NAX:                                                                                    \
+        hipLaunchKernelGGL(k_screen_pre<NAX>, dim3(gp), dim3(1024), 0, st, A, D, K, keep, queue, \
+                           count);                                                               \
         break;
         CMFW_PRE(2) CMFW_PRE(3) CMFW_PRE(4) CMFW_PRE(5) CMFW_PRE(6) CMFW_PRE(7) CMFW_PRE(8)
         CMFW_PRE(9) CMFW_PRE(10) CMFW_PRE(11) CMFW_PRE(12) CMFW_PRE(13) CMFW_PRE(14) CMFW_PRE(15)
