@@ -431,6 +431,126 @@ __global__ __launch_bounds__(256) void k_rns_combine(RnsSegArgs RA) {
     }
 }
 
+// Row-split combine: 8 G lanes per trajectory, lane (r, k) = row r of the
+// running product mod modulus k (rows padded to 8).  A CRT point costs each
+// lane D entries instead of D^2 (Garner over the lanes of moduli 0 and 1 of
+// the same row, the matrix max reduced over the rows); the interval
+// products are D^2 modular products per lane instead of D^3.  Every entry
+// is computed by the operations of k_rns_combine, so the residues, CRT
+// values and results are identical.  For G <= 8 (at most 8 moduli).
+template <int D>
+__global__ __launch_bounds__(256) void k_rns_combine_rows(RnsSegArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G, L = 8 * G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gtid / L, sub = gtid % L;
+    const int r = sub / G, k = sub % G;
+    const int lane = threadIdx.x & 63;
+    const int base = lane - sub;  // the trajectory's first lane in the wave
+    if (t >= A.n) return;         // whole groups leave together
+    const bool kact = k < RA.np;
+    const MontMod M = RA.C.mods[kact ? k : 0];
+    if (RA.nseg > 0 && RA.segV[t] == 2) {  // not an integer point: the sequential kernel walks it
+        if (sub == 0) RA.fallback[t] = 1;
+        return;
+    }
+    if (sub == 0) RA.fallback[t] = 0;
+    const int N = A.N, h = A.h;
+    const int cp1 = N - 2 * h, cp2 = N - h;
+    if (sub == 0) {
+        for (int sn = 0; sn < 3; ++sn) {
+            const int cp = (sn == 0) ? cp1 : (sn == 1) ? cp2 : N;
+            if (cp >= 1 && cp <= N) continue;
+            for (int e = 0; e < D * D; ++e) A.snap[((size_t)(sn * D * D + e)) * A.sld + t] = 0.0;
+            A.snap_ls[(size_t)sn * A.sld + t] = 0.0;
+        }
+    }
+    const int rr = r < D ? r : 0;  // padding rows shadow row 0 and write nothing
+    uint32_t w[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) w[j] = (j == rr) ? M.r1 : 0u;
+    bool ok = true;
+    int nf = 0, ni = 0;
+    double ls = 0.0;
+    const size_t pst = (size_t)A.n * RA.np;  // stride between matrix entries
+    for (int s = 0; ok && s < RA.nseg; ++s) {
+        if (!RA.reduced || s == 0 || RA.crtp[s]) {
+            const size_t o = (size_t)s * A.n + t;
+            nf += RA.segF[o];
+            ni += RA.segI[o];
+            if (RA.segV[o] != 1) {
+                ok = false;
+                break;
+            }
+            // row r of W <- W * P_s (mod m)
+            const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + (size_t)t * RA.np + (kact ? k : 0);
+            uint32_t row[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                uint64_t acc = 0;
+#pragma unroll
+                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)w[kk] * P[(size_t)(kk * D + j) * pst];
+                row[j] = redc(acc, M);
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) w[j] = row[j];
+        }
+        if (!RA.crtp[s + 1]) continue;
+        // CRT point: rns_to_double, snapshot, re-encode (cuda/cmf_walk_rns.cu:285-296)
+        const int step = RA.b[s + 1];
+        uint32_t wn[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) wn[j] = from_mont(w[j], M);
+        if (step == N && RA.resid && kact && r < D) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) RA.resid[((size_t)((k * D + r) * D + j)) * A.n + t] = wn[j];
+        }
+        double wm[D];
+        double mx = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const uint32_t r0 = __shfl(wn[j], base + r * G, 64);
+            const uint32_t r1 = __shfl(wn[j], base + r * G + 1, 64);
+            uint64_t v;
+            wm[j] = crt_garner(r0, r1, RA.np, RA.m0inv, v);
+            if (step == N && RA.crt && k == 0 && r < D) RA.crt[((size_t)(r * D + j)) * A.n + t] = v;
+            const double a = fabs(wm[j]);
+            mx = (a > mx) ? a : mx;
+        }
+        // the matrix max: over the rows of this trajectory's lanes
+        if (r >= D) mx = 0.0;
+#pragma unroll
+        for (int off = 4; off >= 1; off >>= 1) {
+            const double o2 = __shfl_xor(mx, off * G, 64);
+            mx = (o2 > mx) ? o2 : mx;
+        }
+        if ((mx > 0.0) && isfinite(mx)) {
+            const double inv = 1.0 / mx;
+#pragma unroll
+            for (int j = 0; j < D; ++j) wm[j] = wm[j] * inv;
+            ls = ls + log(mx);
+        }
+        for (int sn = 0; sn < 3; ++sn) {
+            const int cp = (sn == 0) ? cp1 : (sn == 1) ? cp2 : N;
+            if (cp != step || k != 0) continue;
+            if (r < D) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) A.snap[((size_t)((sn * D + r) * D + j)) * A.sld + t] = wm[j];
+            }
+            if (r == 0) A.snap_ls[(size_t)sn * A.sld + t] = ls;
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) w[j] = to_mont(to_mod_fast(wm[j], M), M);
+    }
+    if (sub == 0) {
+        if (A.n_fwd) A.n_fwd[t] = nf;
+        if (A.n_inv) A.n_inv[t] = ni;
+        // the snapshot rows were written by the other lanes of this wave
+        __threadfence_block();
+        finish_from_snapshots<D>(A, t, ok);
+    }
+}
+
 template <int D>
 static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
     const int bs = 256;
@@ -450,7 +570,11 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_rns_combine<D>, dim3(gx), dim3(bs), 0, st, RA);
+    if (RA.G <= 8 && (long long)RA.w.n * RA.G * 8 < (1ll << 31))
+        hipLaunchKernelGGL(k_rns_combine_rows<D>, dim3((int)((threads * 8 + bs - 1) / bs)), dim3(bs),
+                           0, st, RA);
+    else
+        hipLaunchKernelGGL(k_rns_combine<D>, dim3(gx), dim3(bs), 0, st, RA);
     return hipGetLastError();
 }
 
