@@ -35,6 +35,7 @@ extern "C" {
 #define CMFW_MAX_DIM 8
 #define CMFW_MAX_AXES 16
 #define CMFW_MAX_PRIMES 64
+#define CMFW_MAX_EXACT_PRIMES 8192
 
 enum {
     CMFW_OK = 0,
@@ -278,6 +279,34 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       (4); every CRT point is a boundary.
  * No counterpart in the reference. */
 int cmfw_set_tuning(const char *key, int64_t value);
+
+/* Exact verification walk ------------------------------------------------
+ * The walk matrix kept exact: the integer-scaled product W' (each forward
+ * axis matrix scaled by a*lead, each inverse by det(a*lead*(I + C/a)), so W'
+ * is the exact walk matrix times a nonzero scalar and every entry ratio --
+ * p/q at a snapshot, every projection of the limit estimate -- is exact),
+ * computed mod K primes below 2^30, one GPU lane per (trajectory, prime).
+ * The caller reconstructs the entries it needs by CRT; they are exact once
+ * the product of the primes exceeds twice their magnitude (use two prime
+ * sets, or K and K - k primes, to see that).  No counterpart in the
+ * reference, whose RNS walk re-encodes a rounded double every 100 steps
+ * (cuda/cmf_walk_rns.cu:285-296); this is what "verified" means here.
+ * Requirements: start points on the integer grid below 2^23 (others get
+ * status 2), z dyadic (z 2^s an integer, s <= 30).
+ * cmfw_exact_primes: the K largest primes below 2^30, descending (host).
+ * cmfw_walk_exact: primes host [K] (odd, < 2^30, pairwise coprime);
+ * snap device u32 [3][rank][rank][n][K]: W' mod prime k at N-2h, N-h, N
+ * (snapshots outside 1..N, and those after a failure, are not written);
+ * status device int32 [n][K]: 1 ok, 0 the walk fails (an exact zero lead or
+ * step denominator), 2 start off the grid, 4 the prime divides the lead or
+ * the solve denominator (a zero solve denominator is exact when every prime
+ * reports 4 at the same step); fail_step device int32 [n][K]: trajectory step
+ * of the failure or -1.  Asynchronous on the stream, except for the upload
+ * of the K moduli's constants. */
+int cmfw_exact_primes(int32_t K, uint32_t *primes);
+int cmfw_walk_exact(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h, int32_t K,
+                    const uint32_t *primes, uint32_t *snap, int32_t *status, int32_t *fail_step,
+                    void *stream);
 
 /* Kernel timing.  With cmfw_set_tuning("kernel_events", 1) the library
  * records a HIP event pair on the launch's stream around each launch of its

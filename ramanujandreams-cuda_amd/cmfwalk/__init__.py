@@ -56,6 +56,10 @@ SIGNATURES = {
     "cmfw_set_tuning": (C.c_int, [C.c_char_p, C.c_int64]),
     "cmfw_kernel_time": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "cmfw_kernel_time_reset": (None, []),
+    "cmfw_exact_primes": (C.c_int, [C.c_int32, C.POINTER(C.c_uint32)]),
+    "cmfw_walk_exact": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
     "cmfw_modulus": (C.c_uint32, [C.c_int32]),
     "cmfw_walk_f64": (C.c_int, [C.POINTER(Desc), C.POINTER(Inputs), C.c_int32, C.c_int32,
                                 C.POINTER(Results), C.POINTER(Snapshots), C.c_void_p, C.c_size_t,

@@ -733,6 +733,67 @@ int cmfw_set_tuning(const char *key, int64_t value) {
     return fail(CMFW_EINVAL, "unknown tuning key %s", key);
 }
 
+int cmfw_exact_primes(int32_t K, uint32_t *primes) {
+    if (K < 1 || K > CMFW_MAX_EXACT_PRIMES || !primes)
+        return fail(CMFW_EINVAL, "exact_primes: K outside 1..%d or NULL output", CMFW_MAX_EXACT_PRIMES);
+    return cmfw::exact_primes(K, primes) == K ? CMFW_OK : fail(CMFW_EINVAL, "exact_primes: too few primes");
+}
+
+int cmfw_walk_exact(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32_t h, int32_t K,
+                    const uint32_t *primes, uint32_t *snap, int32_t *status, int32_t *fail_step,
+                    void *stream) {
+    int rc;
+    if ((rc = check_desc(desc)) || (rc = check_inputs(in))) return rc;
+    if (K < 1 || K > CMFW_MAX_EXACT_PRIMES)
+        return fail(CMFW_EINVAL, "K = %d outside 1..%d", K, CMFW_MAX_EXACT_PRIMES);
+    if (!primes || !snap || !status || !fail_step) return fail(CMFW_EINVAL, "walk_exact: NULL pointer");
+    if (N < 1 || h < 0) return fail(CMFW_EINVAL, "walk_exact: N must be >= 1 and h >= 0");
+    // z must be dyadic: z 2^s an integer for some s <= 30
+    int zshift = -1;
+    int64_t zint = 0;
+    for (int sft = 0; sft <= 30; ++sft) {
+        const double v = ldexp(desc->z, sft);
+        if (v == rint(v) && fabs(v) < 4611686018427387904.0) {
+            zshift = sft;
+            zint = (int64_t)v;
+            break;
+        }
+    }
+    if (zshift < 0) return fail(CMFW_EINVAL, "walk_exact: z = %g is not a dyadic rational", desc->z);
+    for (int k = 0; k < K; ++k)
+        if (primes[k] < 3u || primes[k] >= (1u << 30) || (primes[k] & 1u) == 0u)
+            return fail(CMFW_EINVAL, "walk_exact: modulus %u is not an odd number below 2^30", primes[k]);
+    if (in->n_traj == 0) return CMFW_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t mb = cmfw::exact_mods_bytes(K);
+    std::vector<char> host(mb);
+    cmfw::exact_fill_mods(primes, K, host.data());
+    void *mods = nullptr;
+    hipError_t e = hipMallocAsync(&mods, mb, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(moduli)");
+    e = hipMemcpyAsync(mods, host.data(), mb, hipMemcpyHostToDevice, st);
+    // the host table must outlive the copy
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    cmfw::WalkArgs A;
+    memset(&A, 0, sizeof A);
+    A.p = desc->p;
+    A.q = desc->q;
+    A.naxes = desc->naxes;
+    A.n = in->n_traj;
+    A.ld = in->ld;
+    A.z = desc->z;
+    A.N = N;
+    A.h = h;
+    A.dir = in->dir;
+    A.start = in->start;
+    if (e == hipSuccess)
+        e = cmfw::launch_walk_exact(desc->rank, A, K, zint, zshift, mods, snap, status, fail_step, st);
+    rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "walk_exact launch");
+    hipError_t e2 = hipFreeAsync(mods, st);
+    if (rc == CMFW_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipFreeAsync");
+    return rc;
+}
+
 int cmfw_kernel_time(const char *kernel, double *total_ms, int64_t *launches) {
     if (!kernel || !total_ms || !launches) return fail(CMFW_EINVAL, "kernel_time: null argument");
     std::lock_guard<std::mutex> lk(g_timer_mu);

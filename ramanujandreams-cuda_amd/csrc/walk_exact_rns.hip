@@ -1,0 +1,310 @@
+// walk_exact_rns.hip -- exact verification walk (gfx950).
+//
+// The reference's RNS walk re-encodes a max-normalised double every 100
+// steps (cuda/cmf_walk_rns.cu:285-296), so its "verification" carries FP64
+// rounding.  This walk keeps the walk matrix exact instead: every axis
+// matrix I + C/a (C the theta-companion matrix, cuda/cmf_pfq.cuh:146-291)
+// has rational entries over the integers a (step denominator) and lead
+// (leading coefficient of the theta polynomial), so
+//     S' = a * lead * (I + C/a)
+// is an integer matrix, and the product of the S' (forward steps) and of
+// det(S') * S'^-1 * a * lead (inverse steps) is an integer matrix W' equal
+// to the exact walk matrix times a scalar.  Ratios of entries of W' -- p/q
+// at each snapshot, every projection the limit estimate uses -- are the
+// exact ratios.  W' is computed mod K primes below 2^30, one lane per
+// (trajectory, prime); the host reconstructs the entries it needs by CRT
+// (they are exact once the product of the primes exceeds twice their size).
+//
+// Per axis step and lane: the theta polynomials from the integer
+// coordinates mod p, the column numerators, and the O(d^2) update of the
+// row-vector recurrence; an inverse step solves with modular inverses of a,
+// lead and the solve denominator sd = det(I + C/a), then multiplies by
+// det(S') = (a lead)^d sd.  z must be dyadic (z 2^s an integer, s <= 30):
+// the polynomial D(theta) is scaled by 2^s, which leaves the column unchanged.
+//
+// Failure is decided exactly where it can be: the step denominator and the
+// lead are integers; sd is zero exactly when its numerator c(-a) is, which
+// this walk sees only modulo each prime (status 4 on a lane whose prime
+// divides it; the host decides from all lanes).
+#include "rns_arith.hpp"
+#include "walk_common.hpp"
+#include "walk_kernels.hpp"
+
+namespace cmfw {
+
+struct ExactArgs {
+    WalkArgs w;           // dir, start, ld, n, p, q, naxes, N, h
+    int K;                // primes
+    int64_t zint;         // z = zint / 2^zshift
+    int zshift;
+    const MontMod *mods;  // [K]
+    uint32_t *snap;       // [3][D][D][n][K] residues of W' at N-2h, N-h, N
+    int32_t *status;      // [n][K]: 1 ok, 0 the walk fails (exact), 2 start off the
+                          // integer grid, 4 this prime divides the lead or sd
+    int32_t *fail_step;   // [n][K]: trajectory step of the failure, or -1
+};
+
+__device__ __forceinline__ uint32_t imod_mont(int64_t v, const MontMod &M) {
+    int64_t r = v % (int64_t)M.m;
+    if (r < 0) r += M.m;
+    return to_mont((uint32_t)r, M);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_walk_exact_rns(ExactArgs E) {
+    const WalkArgs &A = E.w;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)A.n * E.K) return;
+    const int t = (int)(gid / E.K), k = (int)(gid % E.K);
+    const MontMod M = E.mods[k];
+    const size_t so = (size_t)t * E.K + k;
+    const int p = A.p, q = A.q, nax = A.naxes;
+    int32_t dir[16], pos[16];
+    bool clean = true;
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+        dir[a] = (a < nax) ? A.dir[(size_t)a * A.ld + t] : 0;
+        const double v = (a < nax) ? A.start[(size_t)a * A.ld + t] : 0.0;
+        clean = clean && v == rint(v) && fabs(v) < 8388608.0;
+        pos[a] = clean ? (int)v : 0;
+        top = max(top, abs(dir[a]));
+    }
+    E.fail_step[so] = -1;
+    if (!clean) {
+        E.status[so] = 2;
+        return;
+    }
+    const uint32_t one = M.r1;
+    uint32_t W[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) W[i][j] = (i == j) ? one : 0u;
+    const uint32_t zm = imod_mont(E.zint, M);
+    const int64_t s2 = (int64_t)1 << E.zshift;
+    const uint32_t sm = imod_mont(s2, M);
+    // The exact lead 2^s D[rank] (the theta polynomials are monic): it moves
+    // with the position only for z = 1, p = q + 1 (rank q: sum(y-1) - sum(x));
+    // otherwise it is 2^s - zint (rank p = q + 1), -zint (rank p > q + 1) or
+    // 2^s (rank q + 1 > p).
+    const bool lead_moves = A.z == 1.0 && p == q + 1 && D == q;
+    const int64_t lead_const = (D == p && p == q + 1) ? s2 - E.zint : (D == p) ? -E.zint : s2;
+    const int N = A.N, h = A.h;
+    const int cps[3] = {N - 2 * h, N - h, N};
+    int status = 1;
+    for (int step = 1; step <= N && status == 1; ++step) {
+        for (int lev = top; lev >= 1 && status == 1; --lev) {
+            for (int ax = nax - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                const bool isx = ax < p;
+                const bool inverse = isx ? (sign < 0) : (sign > 0);
+                const int v = pick(pos, ax);
+                const int a_int = isx ? (inverse ? v - 1 : v) : (inverse ? v : v - 1);
+                // theta polynomials at the evaluation point, mod p
+                uint32_t px[9], py[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) px[c] = py[c] = 0u;
+                px[0] = one;
+                py[0] = one;
+                int64_t sx = 0, sy = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (j < p) {
+                        const int e = pick(pos, j) - ((inverse && isx && j == ax) ? 1 : 0);
+                        sx += e;
+                        const uint32_t em = imod_mont(e, M);
+#pragma unroll
+                        for (int c = j + 1; c >= 1; --c) px[c] = mfma(px[c], em, px[c - 1], M);
+                        px[0] = mmul(px[0], em, M);
+                    }
+                    if (j < q) {
+                        const int e = pick(pos, p + j) - 1 + ((inverse && !isx && j == ax - p) ? 1 : 0);
+                        sy += e;
+                        const uint32_t em = imod_mont(e, M);
+#pragma unroll
+                        for (int c = j + 1; c >= 1; --c) py[c] = mfma(py[c], em, py[c - 1], M);
+                        py[0] = mmul(py[0], em, M);
+                    }
+                }
+                // dcs = 2^s D(theta) = -zint px + 2^s theta py
+                uint32_t dcs[D + 1];
+#pragma unroll
+                for (int c = 0; c <= D; ++c) {
+                    const uint32_t a1 = (c <= p) ? mmul(zm, px[c > 8 ? 8 : c], M) : 0u;
+                    const uint32_t a2 = (c >= 1 && c - 1 <= q) ? mmul(sm, py[(c - 1) > 8 ? 8 : (c - 1)], M) : 0u;
+                    dcs[c] = msub(a2, a1, M.m);
+                }
+                const int64_t lead = lead_moves ? sy - sx : lead_const;
+                if (a_int == 0 || lead == 0) {
+                    status = 0;
+                    E.fail_step[so] = step;
+                    break;
+                }
+                const uint32_t am = imod_mont(a_int, M);  // nonzero: |a| < 2^24 < p
+                const uint32_t lm = dcs[D];
+                if (lm == 0u) {  // the prime divides a nonzero lead
+                    status = 4;
+                    E.fail_step[so] = step;
+                    break;
+                }
+                if (!inverse) {
+                    // W <- W S', S' = a lead I + lead Csub + last column -dcs
+                    const uint32_t al = mmul(am, lm, M);
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        uint64_t acc = 0;
+#pragma unroll
+                        for (int c = 0; c < D; ++c) acc += (uint64_t)W[r][c] * dcs[c];
+                        const uint32_t dot = redc(acc, M);
+#pragma unroll
+                        for (int j = 0; j + 1 < D; ++j) W[r][j] = mmul(lm, mfma(am, W[r][j], W[r][j + 1], M), M);
+                        W[r][D - 1] = msub(mmul(al, W[r][D - 1], M), dot, M.m);
+                    }
+                } else {
+                    // W <- W (I + C/a)^-1 det(S'), C's column col = -dcs / lead
+                    const uint32_t ia = mpow_m2(am, M), il = mpow_m2(lm, M);
+                    uint32_t col[D], be[D];
+#pragma unroll
+                    for (int c = 0; c < D; ++c) col[c] = msub(0u, mmul(dcs[c], il, M), M.m);
+                    be[D - 1] = one;
+#pragma unroll
+                    for (int j = D - 2; j >= 0; --j) be[j] = msub(0u, mmul(be[j + 1], ia, M), M.m);
+                    uint64_t accb = 0;
+#pragma unroll
+                    for (int c = 0; c < D; ++c) accb += (uint64_t)be[c] * col[c];
+                    const uint32_t db = redc(accb, M);
+                    const uint32_t sd = mfma(db, ia, one, M);
+                    if (sd == 0u) {
+                        status = 4;
+                        E.fail_step[so] = step;
+                        break;
+                    }
+                    const uint32_t isd = mpow_m2(sd, M);
+                    uint32_t det = sd, al = mmul(am, lm, M);
+#pragma unroll
+                    for (int c = 0; c < D; ++c) det = mmul(det, al, M);
+#pragma unroll
+                    for (int r = 0; r < D; ++r) {
+                        uint32_t alp[D];
+                        alp[D - 1] = 0u;
+#pragma unroll
+                        for (int j = D - 2; j >= 0; --j) alp[j] = msub(W[r][j], mmul(alp[j + 1], ia, M), M.m);
+                        uint64_t acca = 0;
+#pragma unroll
+                        for (int c = 0; c < D - 1; ++c) acca += (uint64_t)alp[c] * col[c];
+                        const uint32_t da = redc(acca, M);
+                        const uint32_t tt = mmul(msub(W[r][D - 1], mmul(da, ia, M), M.m), isd, M);
+#pragma unroll
+                        for (int j = 0; j < D; ++j) W[r][j] = mmul(mfma(be[j], tt, alp[j], M), det, M);
+                    }
+                }
+                // move the position
+#pragma unroll
+                for (int a = 0; a < 16; ++a) pos[a] += (a == ax) ? sign : 0;
+            }
+        }
+        if (status != 1) break;
+        for (int sn = 0; sn < 3; ++sn) {
+            if (cps[sn] != step) continue;
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                    E.snap[((((size_t)sn * D + i) * D + j) * A.n + t) * E.K + k] = from_mont(W[i][j], M);
+        }
+    }
+    E.status[so] = status;
+}
+
+// The K largest primes below 2^30, descending (deterministic Miller-Rabin
+// for 32-bit integers with bases 2, 7, 61).
+static uint64_t powmod64(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1;
+    b %= m;
+    while (e) {
+        if (e & 1) r = (r * b) % m;
+        b = (b * b) % m;
+        e >>= 1;
+    }
+    return r;
+}
+static bool is_prime32(uint32_t n) {
+    if (n < 2) return false;
+    for (uint32_t s : {2u, 3u, 5u, 7u, 11u, 13u})
+        if (n % s == 0) return n == s;
+    uint32_t d = n - 1;
+    int r = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        ++r;
+    }
+    for (uint32_t a : {2u, 7u, 61u}) {
+        if (a % n == 0) continue;
+        uint64_t x = powmod64(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int i = 1; i < r && comp; ++i) {
+            x = (x * x) % n;
+            if (x == n - 1) comp = false;
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+int exact_primes(int K, uint32_t *out) {
+    int k = 0;
+    for (uint32_t c = (1u << 30) - 1; k < K && c > 3; c -= 2)
+        if (is_prime32(c)) out[k++] = c;
+    return k;
+}
+
+static MontMod mont_of(uint32_t m) {
+    MontMod M;
+    M.m = m;
+    uint32_t inv = 1;
+    for (int i = 0; i < 5; ++i) inv *= 2u - m * inv;
+    M.mp = (uint32_t)(0u - inv);
+    const uint64_t r1 = (((uint64_t)1) << 32) % m;
+    M.r1 = (uint32_t)r1;
+    M.r2 = (uint32_t)((r1 * r1) % m);
+    M.inv = 1.0 / (double)m;
+    return M;
+}
+size_t exact_mods_bytes(int K) { return align256((size_t)K * sizeof(MontMod)); }
+void exact_fill_mods(const uint32_t *primes, int K, void *host_out) {
+    MontMod *o = (MontMod *)host_out;
+    for (int k = 0; k < K; ++k) o[k] = mont_of(primes[k]);
+}
+
+hipError_t launch_walk_exact(int D, const WalkArgs &A, int K, int64_t zint, int zshift,
+                             const void *mods, uint32_t *snap, int32_t *status, int32_t *fail_step,
+                             hipStream_t st) {
+    ExactArgs E;
+    E.w = A;
+    E.K = K;
+    E.zint = zint;
+    E.zshift = zshift;
+    E.mods = (const MontMod *)mods;
+    E.snap = snap;
+    E.status = status;
+    E.fail_step = fail_step;
+    const int bs = 256;
+    const long long threads = (long long)A.n * K;
+    const dim3 grid((unsigned)((threads + bs - 1) / bs));
+    switch (D) {
+        case 2: hipLaunchKernelGGL(k_walk_exact_rns<2>, grid, dim3(bs), 0, st, E); break;
+        case 3: hipLaunchKernelGGL(k_walk_exact_rns<3>, grid, dim3(bs), 0, st, E); break;
+        case 4: hipLaunchKernelGGL(k_walk_exact_rns<4>, grid, dim3(bs), 0, st, E); break;
+        case 5: hipLaunchKernelGGL(k_walk_exact_rns<5>, grid, dim3(bs), 0, st, E); break;
+        case 6: hipLaunchKernelGGL(k_walk_exact_rns<6>, grid, dim3(bs), 0, st, E); break;
+        case 7: hipLaunchKernelGGL(k_walk_exact_rns<7>, grid, dim3(bs), 0, st, E); break;
+        case 8: hipLaunchKernelGGL(k_walk_exact_rns<8>, grid, dim3(bs), 0, st, E); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cmfw

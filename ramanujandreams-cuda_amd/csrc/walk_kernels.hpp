@@ -64,6 +64,13 @@ struct BatchLaunch {
     int maxseg;              // most segments of a plan in this bucket
 };
 size_t batch_bucket_bytes(int D, int count, int maxseg);
+// exact verification walk (walk_exact_rns.hip)
+int exact_primes(int K, uint32_t *out);
+size_t exact_mods_bytes(int K);
+void exact_fill_mods(const uint32_t *primes, int K, void *host_out);
+hipError_t launch_walk_exact(int D, const WalkArgs &A, int K, int64_t zint, int zshift,
+                             const void *mods, uint32_t *snap, int32_t *status, int32_t *fail_step,
+                             hipStream_t st);
 // part: 1 screen + compaction, 2 segment walk, 4 combine + reference-order walks
 hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, int part,
                              hipStream_t st);
