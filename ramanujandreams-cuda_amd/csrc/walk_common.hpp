@@ -276,10 +276,16 @@ __device__ void finish_from_snapshots(const WalkArgs &A, int t, bool ok) {
 // xv/yv hold the evaluation point's x_i and (y_i - 1) values.  Must be
 // compiled without FMA contraction: its bits feed the RNS residues.
 // --------------------------------------------------------------------------
-template <int D>
+// PP > 0: the CMF is PP F (PP-1), known at compile time (p and q are then
+// ignored), so the loops over the parameters unroll without branches.
+template <int D, int PP = 0>
 __device__ __forceinline__ bool companion_column_ref(const double (&xv)[8], const double (&yv)[8],
                                                      int p, int q, double z, double (&col)[D]) {
 #pragma clang fp contract(off)
+    if (PP > 0) {
+        p = PP;
+        q = PP - 1;
+    }
     double px[9], py[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {

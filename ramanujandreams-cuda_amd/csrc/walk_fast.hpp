@@ -108,11 +108,15 @@ __device__ __forceinline__ bool load_fast(const WalkArgs &A, int t, int b0, Fast
 // coordinates, the vanishing test of the lead is the reference's
 // (cuda/cmf_pfq.cuh:146-181; the lead is exact for integer points).  `neg`
 // reports a negative evaluation coordinate.
-template <int D, class Ar>
+template <int D, class Ar, int PP = 0>
 __device__ __forceinline__ bool companion_column_fast(const FastPos &P, int ix, int iy, bool inverse,
                                                       int p, int q, double z,
                                                       typename Ar::T (&col)[D], bool &neg) {
     using T = typename Ar::T;
+    if (PP > 0) {  // PP F (PP-1) known at compile time
+        p = PP;
+        q = PP - 1;
+    }
     T px[9], py[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -241,10 +245,14 @@ struct StepCoef {
     bool inverse;
 };
 
-template <int D, class Ar>
+template <int D, class Ar, int PP = 0>
 __device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, double z, int ax,
                                              int sign, StepCoef<D, Ar> &c, double &tsc) {
     using T = typename Ar::T;
+    if (PP > 0) {
+        p = PP;
+        q = PP - 1;
+    }
     const bool isx = ax < p;
     const int ix = isx ? ax : -1, iy = isx ? -1 : ax - p;
     const bool inverse = isx ? (sign < 0) : (sign > 0);
@@ -252,7 +260,7 @@ __device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, dou
     const int pos = isx ? pick8(P.x, ix) : pick8(P.y, iy);
     const int denom_i = isx ? (inverse ? pos - 1 : pos) : (inverse ? pos : pos - 1);
     bool neg;
-    if (!companion_column_fast<D, Ar>(P, ix, iy, inverse, p, q, z, c.col, neg)) return false;
+    if (!companion_column_fast<D, Ar, PP>(P, ix, iy, inverse, p, q, z, c.col, neg)) return false;
     if (denom_i == 0) return false;  // |denom| <= 64 eps for an integer point
     const double denom = (double)denom_i;
     c.ia = Ar::recip(Ar::from_d(denom));
@@ -286,7 +294,7 @@ __device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, dou
             yv[j] = (double)(P.y[j] - 1 + ((inverse && j == iy) ? 1 : 0));
         }
         double cref[D];
-        if (!companion_column_ref<D>(xv, yv, p, q, z, cref)) return false;
+        if (!companion_column_ref<D, PP>(xv, yv, p, q, z, cref)) return false;
         const double iad = 1.0 / denom;
         double bref[D];
         bref[D - 1] = 1.0;
@@ -339,12 +347,12 @@ __device__ __forceinline__ void step_apply_row(typename Ar::T (&m)[D], const Ste
 
 // One axis step M <- M * (I + C/a)^(+-1) at position P (which the caller
 // moves afterwards).
-template <int D, class Ar>
+template <int D, class Ar, int PP = 0>
 __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const FastPos &P, int p,
                                                int q, double z, int ax, int sign, bool &inverse,
                                                double &tsc) {
     StepCoef<D, Ar> c;
-    const bool ok = step_prepare<D, Ar>(P, p, q, z, ax, sign, c, tsc);
+    const bool ok = step_prepare<D, Ar, PP>(P, p, q, z, ax, sign, c, tsc);
     inverse = c.inverse;
     if (!ok) return false;
 #pragma unroll
@@ -357,9 +365,13 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
 // segment) or 2 (the reference order must decide: start off the integer
 // grid, or a product that vanished or overflowed).  nf / ni count the
 // executed forward / inverse axis steps, the failing one included.
-template <int D, class Ar>
+// PP > 0: the walk is of PP F (PP-1) (A.p, A.q equal to that), known at
+// compile time.
+template <int D, class Ar, int PP = 0>
 __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typename Ar::T (&M)[D][D],
                             int &e2, int &nf, int &ni) {
+    const int wp = PP > 0 ? PP : A.p;
+    const int naxes = PP > 0 ? 2 * PP - 1 : A.naxes;
     FastPos P;
     int top;
     nf = ni = e2 = 0;
@@ -382,13 +394,13 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
     for (int st = 0; ok && st < steps; ++st) {
         const bool watch = D <= kCollapseMaxRank && b0 + st < kCollapseSteps;
         for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+            for (int ax = naxes - 1; ax >= 0; --ax) {
                 const int dv = dir_at(P, ax);
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 bool inverse;
                 double tsc;
-                ok = axis_step_fast<D, Ar>(M, P, A.p, A.q, A.z, ax, sign, inverse, tsc);
+                ok = axis_step_fast<D, Ar, PP>(M, P, A.p, A.q, A.z, ax, sign, inverse, tsc);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 if (!ok) break;
@@ -405,11 +417,11 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
                     }
                 }
                 // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
-                const bool isx = ax < A.p;
+                const bool isx = ax < wp;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     P.x[j] += (isx && j == ax) ? sign : 0;
-                    P.y[j] += (!isx && j == ax - A.p) ? sign : 0;
+                    P.y[j] += (!isx && j == ax - wp) ? sign : 0;
                 }
             }
         }
@@ -546,7 +558,7 @@ __device__ bool combine_segments(const WalkArgs &A, int t, const SegStore<Ar> &S
     return true;
 }
 
-template <int D, class Ar>
+template <int D, class Ar, int PP>
 __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     using T = typename Ar::T;
     const WalkArgs &A = S.w;
@@ -556,7 +568,7 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
     const int t = S.perm ? S.perm[i] : i;
     T M[D][D];
     int e2, nf, ni;
-    const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
+    const int st = walk_segment<D, Ar, PP>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
     const SegStore<Ar> store{S.segM, S.segE, S.segV, S.segF, S.segI, (size_t)A.n};
     store_segment<D, Ar>(store, s, (size_t)i, st, M, e2, nf, ni);
 }
@@ -567,7 +579,7 @@ __global__ __launch_bounds__(256) void k_walk_seg(SegArgsT<Ar> S) {
 // different trajectories would execute the union of their sequences (about
 // twice the steps for directions drawn independently per axis).  For walks
 // too small to be ordered by direction.
-template <int D, class Ar>
+template <int D, class Ar, int PP>
 __global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
     using T = typename Ar::T;
     const WalkArgs &A = S.w;
@@ -577,7 +589,7 @@ __global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
     const int t = S.perm ? S.perm[i] : i;
     T M[D][D];
     int e2, nf, ni;
-    const int st = walk_segment<D, Ar>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
+    const int st = walk_segment<D, Ar, PP>(A, t, S.b[s], S.b[s + 1] - S.b[s], M, e2, nf, ni);
     // trajectory-contiguous store (item-major: [trajectory][segment][D][D]),
     // so the lanes of a wave fill whole cache lines; k_walk_combine_rows
     // reads this layout
@@ -837,15 +849,15 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
     finish_from_snapshots<D>(A, t, ok);
 }
 
-template <int D, class Ar>
-hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
+template <int D, class Ar, int PP>
+hipError_t launch_fast_pp(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     const int bs = 256;
     const int gx = (S.w.n + bs - 1) / bs;
     constexpr int G = D <= 2 ? 2 : D <= 4 ? 4 : 8;
     const bool rows = layout == kSegRows;
     hipEvent_t t0 = kernel_timer_begin(st);
     if (layout == kSegTrajMajor)
-        hipLaunchKernelGGL((k_walk_seg_tm<D, Ar>),
+        hipLaunchKernelGGL((k_walk_seg_tm<D, Ar, PP>),
                            dim3((unsigned)(((size_t)S.w.n * S.nseg + bs - 1) / bs)), dim3(bs), 0,
                            st, S);
     else if (rows)
@@ -853,7 +865,7 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
                            dim3((unsigned)(((size_t)S.w.n * G + bs - 1) / bs), S.nseg), dim3(bs), 0,
                            st, S);
     else
-        hipLaunchKernelGGL((k_walk_seg<D, Ar>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
+        hipLaunchKernelGGL((k_walk_seg<D, Ar, PP>), dim3(gx, S.nseg), dim3(bs), 0, st, S);
     kernel_timer_end(Ar::kSegKernel, t0, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -863,6 +875,16 @@ hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     else
         hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
     return hipGetLastError();
+}
+
+// The pFq family q = p - 1 (rank p, or p - 1 at z = 1) gets kernels with p
+// known at compile time; any other (p, q) the generic ones.
+template <int D, class Ar>
+hipError_t launch_fast_d(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
+    constexpr int P1 = D + 1 <= 8 ? D + 1 : 0;
+    if (S.w.q == S.w.p - 1 && S.w.p == D) return launch_fast_pp<D, Ar, D>(S, layout, st);
+    if (P1 > 0 && S.w.q == S.w.p - 1 && S.w.p == P1) return launch_fast_pp<D, Ar, P1>(S, layout, st);
+    return launch_fast_pp<D, Ar, 0>(S, layout, st);
 }
 
 template <class Ar>
