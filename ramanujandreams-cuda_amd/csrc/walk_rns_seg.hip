@@ -67,9 +67,12 @@ __device__ __forceinline__ bool integer_point(double v) {
     return v == rint(v) && fabs(v) < 1073741824.0;
 }
 
-template <int D>
+template <int D, int PP>
 __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
     const WalkArgs &A = RA.w;
+    // PP > 0: PP F (PP-1) known at compile time
+    const int wp = PP > 0 ? PP : A.p, wq = PP > 0 ? PP - 1 : A.q;
+    const int wnax = PP > 0 ? 2 * PP - 1 : A.naxes;
     const int G = RA.G;
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = gtid % G;
@@ -129,16 +132,16 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
     const int steps = RA.b[s + 1] - RA.b[s];
     for (int st = 0; ok && st < steps; ++st) {
         for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+            for (int ax = wnax - 1; ax >= 0; --ax) {
                 const int dv = pick(dir, ax);
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 double xv[8], yv[8], denom;
-                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                const bool inverse = axis_eval(xs, ys, wp, ax, sign, xv, yv, denom);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 double col[D];
-                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col);
+                ok = companion_column_ref<D, PP>(xv, yv, wp, wq, A.z, col);
                 ok = ok && isfinite(denom) && !(fabs(denom) <= 64.0 * kEps);
                 if (!ok) break;
                 uint32_t cr[D];
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
                         for (int j = 0; j < D; ++j) R[r][j] = mfma(be[j], tt, al[j], M);
                     }
                 }
-                axis_move(xs, ys, A.p, ax, sign);
+                axis_move(xs, ys, wp, ax, sign);
             }
         }
     }
@@ -558,11 +561,15 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
     const int gx = (int)((threads + bs - 1) / bs);
     if (RA.nseg > 0) {
         hipEvent_t t0 = kernel_timer_begin(st);
-        if (RA.tm)
-            hipLaunchKernelGGL(k_rns_segment<D>, dim3((int)((threads * RA.nseg + bs - 1) / bs)),
-                               dim3(bs), 0, st, RA);
+        const dim3 grid = RA.tm ? dim3((int)((threads * RA.nseg + bs - 1) / bs)) : dim3(gx, RA.nseg);
+        // p known at compile time for the q = p - 1 family
+        constexpr int P1 = D + 1 <= 8 ? D + 1 : 0;
+        if (RA.w.q == RA.w.p - 1 && RA.w.p == D)
+            hipLaunchKernelGGL((k_rns_segment<D, D>), grid, dim3(bs), 0, st, RA);
+        else if (P1 > 0 && RA.w.q == RA.w.p - 1 && RA.w.p == P1)
+            hipLaunchKernelGGL((k_rns_segment<D, P1>), grid, dim3(bs), 0, st, RA);
         else
-            hipLaunchKernelGGL(k_rns_segment<D>, dim3(gx, RA.nseg), dim3(bs), 0, st, RA);
+            hipLaunchKernelGGL((k_rns_segment<D, 0>), grid, dim3(bs), 0, st, RA);
         kernel_timer_end("rns_segment", t0, st);
         if (RA.reduced)
             hipLaunchKernelGGL(k_rns_reduce<D>, dim3((int)((threads * RA.ncrt + bs - 1) / bs)),

@@ -12,9 +12,12 @@ namespace cmfw {
 // cuda/cmf_pfq.cuh:146-235); nf / ni count the evaluated forward / inverse
 // axis steps, the failing one included.  Start points off the integer grid
 // (or beyond 2^23) pass with no step evaluated.
-template <int D>
+template <int D, int PP = 0>
 __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
 #pragma clang fp contract(off)
+    // PP > 0: PP F (PP-1) known at compile time
+    const int wp = PP > 0 ? PP : A.p, wq = PP > 0 ? PP - 1 : A.q;
+    const int wnax = PP > 0 ? 2 * PP - 1 : A.naxes;
     int32_t dir[16];
     double xs[8], ys[8];
     load_traj(A, t, dir, xs, ys);
@@ -32,16 +35,16 @@ __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
     bool ok = true;
     for (int step = 1; ok && step <= K; ++step) {
         for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+            for (int ax = wnax - 1; ax >= 0; --ax) {
                 const int dv = pick(dir, ax);
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 double xv[8], yv[8], denom;
-                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                const bool inverse = axis_eval(xs, ys, wp, ax, sign, xv, yv, denom);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 double col[D];
-                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col) && isfinite(denom) &&
+                ok = companion_column_ref<D, PP>(xv, yv, wp, wq, A.z, col) && isfinite(denom) &&
                      !(fabs(denom) <= 64.0 * kEps);
                 if (ok && inverse) {
                     const double ia = 1.0 / denom;
@@ -57,7 +60,7 @@ __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
                     ok = isfinite(sd) && !(fabs(sd) <= 128.0 * kEps * ss);
                 }
                 if (!ok) break;
-                axis_move(xs, ys, A.p, ax, sign);
+                axis_move(xs, ys, wp, ax, sign);
             }
         }
     }
