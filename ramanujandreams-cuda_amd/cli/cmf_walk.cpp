@@ -229,9 +229,10 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     // the reference recurrence per executed step (F_fwd = C + 5d^2 + d + 2,
     // F_inv = C + 10d^2 + 3d + 2, C = 1.5p(p+1) + 1.5q(q+1) + p + q + 2)
     // against the FP64 vector peak (78.6 TFLOP/s); rns: modular products of
-    // the reference algorithm per lane (forward 2d^2 + one Fermat power,
-    // inverse d(5d+1) + (d+1) powers, 45 products a power) against 3.3e12/s
-    // (quarter-rate 32-bit multiplies, three a Montgomery product).
+    // the matrix recurrence per lane (forward 2d^2, inverse d(5d+1); the
+    // modular inverses are not counted: their cost depends on the method, a
+    // Fermat power in the reference, mostly a table lookup here) against
+    // 3.3e12/s (quarter-rate 32-bit multiplies, three a Montgomery product).
     double sf = 0, si = 0;
     for (int i = 0; i < n; ++i) {
         sf += nfw[i];
@@ -241,7 +242,7 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     double work, peak;
     const char *unit;
     if (is_rns()) {
-        work = n_primes * (sf * (2 * d * d + 45) + si * (d * (5 * d + 1) + (d + 1) * 45));
+        work = n_primes * (sf * (2 * d * d) + si * (d * (5 * d + 1)));
         peak = 256 * 2.4e9 * 16 / 3;
         unit = "modmul";
     } else {
