@@ -612,16 +612,34 @@ __global__ __launch_bounds__(256) void k_walk_seg_tm(SegArgsT<Ar> S) {
 // group.  Same arithmetic, same result as k_walk_seg, with several times
 // the resident waves: the choice for batches too small to fill the GPU one
 // lane per (trajectory, segment).
+// Group max over G <= 8 consecutive lanes by DPP lane moves (VALU
+// latency, no LDS round trip): swap neighbours (quad_perm [1,0,3,2]), swap
+// pairs (quad_perm [2,3,0,1]), then mirror each half-row of 8 lanes, which
+// pairs the two quads.  Every lane of a group must be active.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i32<CTRL>((int)(b & 0xffffffffll)), hi = dpp_i32<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
 template <int G>
 __device__ __forceinline__ double group_max(double v) {
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, G));
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8, "groups of 1, 2, 4 or 8 lanes");
+    if (G >= 2) v = fmax(v, dpp_f64<0xB1>(v));
+    if (G >= 4) v = fmax(v, dpp_f64<0x4E>(v));
+    if (G >= 8) v = fmax(v, dpp_f64<0x141>(v));
     return v;
 }
 template <int G>
 __device__ __forceinline__ float group_max(float v) {
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, G));
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8, "groups of 1, 2, 4 or 8 lanes");
+    if (G >= 2) v = fmaxf(v, __int_as_float(dpp_i32<0xB1>(__float_as_int(v))));
+    if (G >= 4) v = fmaxf(v, __int_as_float(dpp_i32<0x4E>(__float_as_int(v))));
+    if (G >= 8) v = fmaxf(v, __int_as_float(dpp_i32<0x141>(__float_as_int(v))));
     return v;
 }
 
