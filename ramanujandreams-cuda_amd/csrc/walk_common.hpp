@@ -30,8 +30,9 @@ __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size
 
 // Depth-segment plan of the fast engines.  The boundaries are the
 // multiples of L and the snapshot depths N-2h, N-h, N; L is about
-// N / target (at least min_steps) and doubles until there are at most
-// kMaxSegments segments.  It depends on N and h only, so a trajectory's
+// N / target, at least min_steps, or N / 20 for short walks (so a walk of
+// N < 20 min_steps still has about 20 segments to spread over lanes), and
+// doubles until there are at most kMaxSegments segments.  It depends on N and h only, so a trajectory's
 // fast result does not depend on the batch it is walked in.  Host and
 // device evaluate the same function.
 struct FastPlan {
@@ -51,7 +52,8 @@ __host__ __device__ inline int fast_plan_next(const FastPlan &P, int prev) {
 __host__ __device__ inline FastPlan fast_plan(int N, int h, int target, int min_steps) {
     FastPlan P{N, N - 2 * h, N - h, 1};
     int L = (N + target - 1) / target;
-    if (L < min_steps) L = min_steps;
+    const int ms = (N / 20 < min_steps) ? (N / 20 > 0 ? N / 20 : 1) : min_steps;
+    if (L < ms) L = ms;
     if (L < 1) L = 1;
     for (;;) {
         P.L = L;
