@@ -132,13 +132,12 @@ class Search:
         self.n_primes = n_primes
 
     def screen_compact(self, T):
-        """Exact screen, then the survivors sorted by direction so that the
-        lanes of a wave follow the same axis sequence."""
-        torch = self.torch
+        """Exact screen, then the survivors' directions and starts gathered
+        in candidate order.  (Sorting them by direction first would make the
+        lanes of a feature-walk wave share more of their axis sequence: 0.05
+        ms saved in that walk for 0.34 ms spent sorting.)"""
         live = self.pr.screen(P, Q, Z, self.dir, self.start, keep=self.keep)
         T.mark("screen")
-        key = self.pr.direction_keys(self.dir[:, live])
-        live = live[torch.argsort(key)]
         dl = self.dir[:, live].contiguous()
         sl = self.start[:, live].contiguous()
         T.mark("compact")
@@ -175,9 +174,7 @@ class Search:
         s = net.score(X)
         m = min(M, live.numel())
         cand = torch.topk(s, m).indices if m > 0 else live[:0]
-        # back to direction order: the lanes of a wave then share their
-        # axis sequence (no divergence)
-        cand = torch.sort(cand).values
+        cand = torch.sort(cand).values  # candidate order
         T.mark("rank")
         res = pr.WALKS[engine](P, Q, Z, dl[:, cand].contiguous(), sl[:, cand].contiguous(),
                                N_DEPTH, H)
