@@ -4,7 +4,8 @@
 // creation the planner
 //   1. validates every trajectory and computes its rank D (make_cmf_desc's
 //      rule, cuda/cmf_pfq.cuh:41-61);
-//   2. buckets the trajectories by D and orders each bucket by depth,
+//   2. buckets the trajectories by D and CMF class (p known at compile
+//      time for the pFq family q = p - 1) and orders each bucket by depth,
 //      longest first, into a device work queue;
 //   3. optionally autotunes each bucket's launch shape (block size and
 //      resident waves per SIMD of the persistent segment kernel) by timing
@@ -38,7 +39,15 @@ int set_hip_error(hipError_t e, const char *what);
 
 namespace {
 
-constexpr int kMinD = 2, kMaxD = 8, kBuckets = kMaxD - kMinD + 1;
+// Buckets: rank D x CMF class -- p = D and q = p - 1 (class 0), p = D + 1
+// and q = p - 1 (class 1; the z = 1 rank drop), any other (p, q) (class 2).
+// Classes 0 and 1 run kernels with p known at compile time.
+constexpr int kMinD = 2, kMaxD = 8, kClasses = 3, kBuckets = (kMaxD - kMinD + 1) * kClasses;
+inline int bucket_rank(int k) { return k / kClasses + kMinD; }
+inline int bucket_p(int k) {
+    const int D = bucket_rank(k), c = k % kClasses;
+    return c == 0 ? D : (c == 1 && D + 1 <= 8) ? D + 1 : 0;
+}
 
 struct Shape {
     int block = 256;
@@ -107,7 +116,8 @@ hipError_t launch_bucket(cmfw_batch *b, int k, int part, hipStream_t st) {
     L.target = b->target;
     L.min_steps = b->min_steps;
     L.maxseg = b->maxseg[k];
-    return cmfw::launch_walk_batch(k + kMinD, L, s.block, grid_of(b, s), part, st);
+    L.pp = bucket_p(k);
+    return cmfw::launch_walk_batch(bucket_rank(k), L, s.block, grid_of(b, s), part, st);
 }
 
 // Times the candidate shapes of bucket k's persistent segment kernel on
@@ -231,7 +241,8 @@ int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int3
             snprintf(msg, sizeof msg, "trajectory %d: rank %d outside 2..8", t, D);
             return cmfw::set_error(CMFW_EINVAL, msg);
         }
-        bucket[D - kMinD].push_back(t);
+        const int cls = (q == p - 1 && p == D) ? 0 : (q == p - 1 && p == D + 1) ? 1 : 2;
+        bucket[(D - kMinD) * kClasses + cls].push_back(t);
     }
     cmfw_batch *b = new cmfw_batch;
     b->n = n;
@@ -272,7 +283,7 @@ int cmfw_batch_create(const cmfw_batch_spec *spec, const cmfw_results *out, int3
                                                                         b->min_steps)));
         }
         b->maxseg[k] = ms;
-        if (count > 0) e = hipMalloc(&b->ws[k], cmfw::batch_bucket_bytes(k + kMinD, count, ms));
+        if (count > 0) e = hipMalloc(&b->ws[k], cmfw::batch_bucket_bytes(bucket_rank(k), count, ms));
     }
     // snapshots: each bucket indexes [3][D][D][n] with its own D; buckets
     // hold disjoint trajectories, so one buffer sized for D = 8 serves all
@@ -338,9 +349,9 @@ int cmfw_batch_info(const cmfw_batch *b, char *buf, size_t len) {
         if (count == 0) continue;
         char item[256];
         snprintf(item, sizeof item,
-                 "%s{\"rank\": %d, \"count\": %d, \"block\": %d, \"waves_per_simd\": %d, "
-                 "\"grid\": %d, \"tuned_sample_ms\": %.4f}",
-                 first ? "" : ", ", k + kMinD, count, b->shape[k].block,
+                 "%s{\"rank\": %d, \"p\": %d, \"count\": %d, \"block\": %d, "
+                 "\"waves_per_simd\": %d, \"grid\": %d, \"tuned_sample_ms\": %.4f}",
+                 first ? "" : ", ", bucket_rank(k), bucket_p(k), count, b->shape[k].block,
                  b->shape[k].waves_per_simd, grid_of(b, b->shape[k]), b->shape[k].tuned_ms);
         s += item;
         first = false;

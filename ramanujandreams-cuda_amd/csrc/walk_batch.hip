@@ -58,14 +58,14 @@ __device__ __forceinline__ bool walk_valid_depth(const WalkArgs &A) {
     return !(A.N <= 0 || A.h <= 0 || A.N - 2 * A.h <= 0);
 }
 
-template <int D>
+template <int D, int PP>
 __global__ __launch_bounds__(256) void k_batch_screen(BatchArgs B) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.count) return;
     const int t = B.queue[i];
     const WalkArgs A = traj_args(B, t);
     int nf = 0, ni = 0;
-    const bool ok = walk_valid_depth(A) && screen_traj<D>(A, t, 1, nf, ni);
+    const bool ok = walk_valid_depth(A) && screen_traj<D, PP>(A, t, 1, nf, ni);
     B.keep[i] = ok ? 1 : 0;
     if (ok) return;
     // the reference fails at this check: same counts, invalid result
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(1024) void k_batch_compact(BatchArgs B) {
     if (tid == 0) *B.nkept = carry;
 }
 
-template <int D>
+template <int D, int PP>
 __global__ __launch_bounds__(256) void k_batch_seg(BatchArgs B) {
     const int lane = threadIdx.x & 63;
     const int nk = *B.nkept;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_batch_seg(BatchArgs B) {
         if (b0 >= A.N) continue;  // the plan has fewer segments
         ArithF64::T M[D][D];
         int e2, nf, ni;
-        const int st = walk_segment<D, ArithF64>(A, t, b0, b1 - b0, M, e2, nf, ni);
+        const int st = walk_segment<D, ArithF64, PP>(A, t, b0, b1 - b0, M, e2, nf, ni);
         store_segment<D, ArithF64>(B.seg, s, (size_t)j, st, M, e2, nf, ni);
     }
 }
@@ -143,12 +143,12 @@ __global__ __launch_bounds__(256) void k_batch_combine(BatchArgs B) {
     B.exact[j] = ok ? 0 : 1;
 }
 
-template <int D>
+template <int D, int PP>
 __global__ __launch_bounds__(256) void k_batch_exact(BatchArgs B) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= *B.nkept || !B.exact[j]) return;
     const int t = B.queue[B.kept[j]];
-    walk_exact_traj<D>(traj_args(B, t), t);
+    walk_exact_traj<D, PP>(traj_args(B, t), t);
 }
 
 __global__ void k_zero_i32(int32_t *p, int n) {
@@ -156,26 +156,37 @@ __global__ void k_zero_i32(int32_t *p, int n) {
     if (i < n) p[i] = 0;
 }
 
-template <int D>
-static hipError_t launch_batch_d(const BatchArgs &B, int block, int grid, int part,
-                                 hipStream_t st) {
+template <int D, int PP>
+static hipError_t launch_batch_pp(const BatchArgs &B, int block, int grid, int part,
+                                  hipStream_t st) {
     const int g = (B.count + 255) / 256;
     hipError_t e = hipSuccess;
     if (part & 1) {
-        hipLaunchKernelGGL(k_batch_screen<D>, dim3(g), dim3(256), 0, st, B);
+        hipLaunchKernelGGL((k_batch_screen<D, PP>), dim3(g), dim3(256), 0, st, B);
         hipLaunchKernelGGL(k_batch_compact, dim3(1), dim3(1024), 0, st, B);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (part & 2) {
-        hipLaunchKernelGGL(k_batch_seg<D>, dim3(grid), dim3(block), 0, st, B);
+        hipLaunchKernelGGL((k_batch_seg<D, PP>), dim3(grid), dim3(block), 0, st, B);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (part & 4) {
         hipLaunchKernelGGL(k_batch_combine<D>, dim3(g), dim3(256), 0, st, B);
-        hipLaunchKernelGGL(k_batch_exact<D>, dim3(g), dim3(256), 0, st, B);
+        hipLaunchKernelGGL((k_batch_exact<D, PP>), dim3(g), dim3(256), 0, st, B);
         e = hipGetLastError();
     }
     return e;
+}
+
+// pp: the bucket's p when every trajectory in it is pp F (pp-1) (pp = D or
+// D + 1), else 0
+template <int D>
+static hipError_t launch_batch_d(const BatchArgs &B, int pp, int block, int grid, int part,
+                                 hipStream_t st) {
+    constexpr int P1 = D + 1 <= 8 ? D + 1 : 0;
+    if (pp == D) return launch_batch_pp<D, D>(B, block, grid, part, st);
+    if (P1 > 0 && pp == P1) return launch_batch_pp<D, P1>(B, block, grid, part, st);
+    return launch_batch_pp<D, 0>(B, block, grid, part, st);
 }
 
 size_t batch_bucket_bytes(int D, int count, int maxseg) {
@@ -221,13 +232,13 @@ hipError_t launch_walk_batch(int D, const BatchLaunch &L, int block, int grid, i
     p += align256(n * 4);
     B.nkept = (int32_t *)p;
     switch (D) {
-        case 2: return launch_batch_d<2>(B, block, grid, part, st);
-        case 3: return launch_batch_d<3>(B, block, grid, part, st);
-        case 4: return launch_batch_d<4>(B, block, grid, part, st);
-        case 5: return launch_batch_d<5>(B, block, grid, part, st);
-        case 6: return launch_batch_d<6>(B, block, grid, part, st);
-        case 7: return launch_batch_d<7>(B, block, grid, part, st);
-        case 8: return launch_batch_d<8>(B, block, grid, part, st);
+        case 2: return launch_batch_d<2>(B, L.pp, block, grid, part, st);
+        case 3: return launch_batch_d<3>(B, L.pp, block, grid, part, st);
+        case 4: return launch_batch_d<4>(B, L.pp, block, grid, part, st);
+        case 5: return launch_batch_d<5>(B, L.pp, block, grid, part, st);
+        case 6: return launch_batch_d<6>(B, L.pp, block, grid, part, st);
+        case 7: return launch_batch_d<7>(B, L.pp, block, grid, part, st);
+        case 8: return launch_batch_d<8>(B, L.pp, block, grid, part, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -84,9 +84,12 @@ __device__ __forceinline__ bool right_mul_ref(double (&M)[D][D], const double (&
 
 // The whole walk of trajectory t (forward_walk, cuda/cmf_pfq.cuh:348-371)
 // followed by the snapshot reductions; in prefix mode only A.prefix steps.
-template <int D>
+// PP > 0: the walk is of PP F (PP-1), known at compile time.
+template <int D, int PP = 0>
 __device__ void walk_exact_traj(const WalkArgs &A, int t) {
 #pragma clang fp contract(off)
+    const int wp = PP > 0 ? PP : A.p, wq = PP > 0 ? PP - 1 : A.q;
+    const int wnax = PP > 0 ? 2 * PP - 1 : A.naxes;
     int32_t dir[16];
     double xs[8], ys[8];
     load_traj(A, t, dir, xs, ys);
@@ -108,19 +111,19 @@ __device__ void walk_exact_traj(const WalkArgs &A, int t) {
     for (int step = 1; ok && step <= N; ++step) {
         // apply_trajectory_step (cuda/cmf_pfq.cuh:305-322)
         for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = A.naxes - 1; ax >= 0; --ax) {
+            for (int ax = wnax - 1; ax >= 0; --ax) {
                 const int dv = pick(dir, ax);
                 if (abs(dv) < lev) continue;
                 const int sign = dv > 0 ? 1 : -1;
                 double xv[8], yv[8], denom;
-                const bool inverse = axis_eval(xs, ys, A.p, ax, sign, xv, yv, denom);
+                const bool inverse = axis_eval(xs, ys, wp, ax, sign, xv, yv, denom);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
                 double col[D];
-                ok = companion_column_ref<D>(xv, yv, A.p, A.q, A.z, col) &&
+                ok = companion_column_ref<D, PP>(xv, yv, wp, wq, A.z, col) &&
                      right_mul_ref<D>(M, col, denom, inverse, ls);
                 if (!ok) break;
-                axis_move(xs, ys, A.p, ax, sign);
+                axis_move(xs, ys, wp, ax, sign);
             }
         }
         if (!ok) break;
@@ -144,11 +147,11 @@ __device__ void walk_exact_traj(const WalkArgs &A, int t) {
     finish_from_snapshots<D>(A, t, ok);
 }
 
-template <int D>
+template <int D, int PP>
 __global__ __launch_bounds__(256) void k_walk_f64_ref(WalkArgs A) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.n) return;
-    walk_exact_traj<D>(A, t);
+    walk_exact_traj<D, PP>(A, t);
 }
 
 }  // namespace cmfw

@@ -10,11 +10,18 @@
 
 namespace cmfw {
 
+// p known at compile time for the pFq family q = p - 1 (rank p or p - 1)
 template <int D>
 static hipError_t launch_ref(const WalkArgs &A, hipStream_t st) {
     const int bs = 256;
     const int grid = (A.n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_walk_f64_ref<D>, dim3(grid), dim3(bs), 0, st, A);
+    constexpr int P1 = D + 1 <= 8 ? D + 1 : 0;
+    if (A.q == A.p - 1 && A.p == D)
+        hipLaunchKernelGGL((k_walk_f64_ref<D, D>), dim3(grid), dim3(bs), 0, st, A);
+    else if (P1 > 0 && A.q == A.p - 1 && A.p == P1)
+        hipLaunchKernelGGL((k_walk_f64_ref<D, P1>), dim3(grid), dim3(bs), 0, st, A);
+    else
+        hipLaunchKernelGGL((k_walk_f64_ref<D, 0>), dim3(grid), dim3(bs), 0, st, A);
     return hipGetLastError();
 }
 

@@ -62,6 +62,7 @@ struct BatchLaunch {
     void *ws;                // batch_bucket_bytes(D, count, maxseg)
     int target, min_steps;   // fast plan parameters
     int maxseg;              // most segments of a plan in this bucket
+    int pp;                  // p of every trajectory (all pp F (pp-1)), or 0
 };
 size_t batch_bucket_bytes(int D, int count, int maxseg);
 // exact verification walk (walk_exact_rns.hip)
