@@ -184,11 +184,14 @@ static int run_benchmark(int p, int q, double z, int N, int h, int n_traj, int m
     R.ws_bytes = cmfw_workspace_bytes(&desc, n, N, h, n_primes);
     R.ws = R.alloc<char>(R.ws_bytes);
 
-    // warm-up on one trajectory, as the reference does before timing
+    // Warm-up before timing, as the reference does (cuda/cmf_walk_f64.cu:214-215
+    // warms up on one trajectory).  Here it walks the whole input: the
+    // engine's kernels and layouts depend on the trajectory count (a single
+    // walk takes the trajectory-major layout and skips the direction sort),
+    // and HIP loads a kernel's code object on its first launch, so a
+    // one-trajectory warm-up would leave that loading inside the timed run.
     if (n > 0) {
-        cmfw_inputs one = R.in;
-        one.n_traj = 1;
-        if (launch(desc, one, N, h, n_primes, R.res, R.ws, R.ws_bytes) != CMFW_OK) {
+        if (launch(desc, R.in, N, h, n_primes, R.res, R.ws, R.ws_bytes) != CMFW_OK) {
             fprintf(stderr, "walk failed: %s\n", cmfw_last_error());
             return 2;
         }

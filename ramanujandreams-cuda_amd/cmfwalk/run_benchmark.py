@@ -19,6 +19,9 @@ achieved rate and its fraction of the MI355X peak.
 python -m cmfwalk.run_benchmark [--bin-dir DIR] [--output-dir DIR] [--N 1000]
     [--h 50] [--n-traj 10000] [--z 0.5] [--cmfs ...] [--variants ...]
     [--modes ...] [--n-primes 8] [--zeta]
+
+Variant "f64_fast" (not in the reference) runs cmf_walk_f64 --fast, the
+fast FP64 engine; its rows are labelled f64_fast.
 """
 import argparse
 import json
@@ -39,10 +42,13 @@ CSV_COLUMNS = ("cmf,variant,mode,rank,N,h,n_traj,gpu_time_ms,throughput_traj_s,"
 ROOFLINE_COLUMNS = "n_fwd_steps,n_inv_steps,work_unit,canonical_work,achieved_per_s,roofline_frac"
 
 
-def run_one(binary, cmf, z, N, h, n_traj, mode, output_path, n_primes=None, use_zeta=False):
+def run_one(binary, cmf, z, N, h, n_traj, mode, output_path, n_primes=None, use_zeta=False,
+            fast=False):
     """One executable run; its JSON plus wall_time_s, or None on failure."""
     cmd = [str(binary), "--cmf", cmf, "--z", str(z), "--N", str(N), "--h", str(h),
            "--n-traj", str(n_traj), "--mode", mode, "--output", str(output_path)]
+    if fast:
+        cmd.append("--fast")
     if n_primes and "rns" in Path(binary).name:
         cmd += ["--n-primes", str(n_primes)]
     if use_zeta:
@@ -126,7 +132,8 @@ def main(argv=None):
     k = 0
     for cmf in args.cmfs:
         for variant in args.variants:
-            binary = args.bin_dir / f"cmf_walk_{variant}"
+            fast = variant == "f64_fast"
+            binary = args.bin_dir / ("cmf_walk_f64" if fast else f"cmf_walk_{variant}")
             if not binary.exists():
                 print(f"  SKIP: {binary} not found", flush=True)
                 continue
@@ -135,7 +142,7 @@ def main(argv=None):
                 print(f"\n[{k}/{total}] {cmf} / {variant} / {mode}", flush=True)
                 out = args.output_dir / "individual" / f"{cmf}_{variant}_{mode}.json"
                 r = run_one(binary, cmf, args.z, args.N, args.h, args.n_traj, mode, out,
-                            n_primes=args.n_primes, use_zeta=args.zeta)
+                            n_primes=args.n_primes, use_zeta=args.zeta, fast=fast)
                 if r:
                     r["cmf"], r["variant"], r["mode"] = cmf, variant, mode
                     results.append(r)

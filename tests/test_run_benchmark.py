@@ -37,15 +37,40 @@ def test_tables_keep_reference_columns(tmp_path):
     assert "| 3F2 | f64 | limit | 3 |" in md and "| 4/4 |" in md
 
 
+def test_fast_variant_runs_f64_binary_with_fast_flag(tmp_path, monkeypatch):
+    from cmfwalk import run_benchmark as rb
+
+    calls = []
+
+    def fake_run(cmd, capture_output, text):
+        calls.append(cmd)
+        out = cmd[cmd.index("--output") + 1]
+        with open(out, "w") as f:
+            json.dump({"gpu_time_ms": 1.0, "throughput_traj_per_s": 1.0}, f)
+        return subprocess.CompletedProcess(cmd, 0, "", "")
+
+    monkeypatch.setattr(rb.subprocess, "run", fake_run)
+    bins = tmp_path / "bin"
+    bins.mkdir()
+    for v in ("f64", "rns", "df64"):
+        (bins / f"cmf_walk_{v}").write_text("")
+    assert rb.main(["--bin-dir", str(bins), "--output-dir", str(tmp_path / "o"), "--cmfs", "3F2",
+                    "--variants", "f64", "f64_fast", "--modes", "limit"]) == 0
+    assert [os.path.basename(c[0]) for c in calls] == ["cmf_walk_f64", "cmf_walk_f64"]
+    assert "--fast" not in calls[0] and "--fast" in calls[1]
+    res = json.load(open(tmp_path / "o" / "summary" / "all_results.json"))
+    assert [r["variant"] for r in res] == ["f64", "f64_fast"]
+
+
 @pytest.mark.gpu
 def test_runner_end_to_end(tmp_path):
     r = subprocess.run([sys.executable, "-m", "cmfwalk.run_benchmark", "--output-dir",
                         str(tmp_path), "--cmfs", "3F2", "6F5", "--N", "200", "--h", "50",
-                        "--n-traj", "2000"], cwd=PKG, capture_output=True, text=True,
-                       timeout=300)
+                        "--n-traj", "2000", "--variants", "f64", "f64_fast", "rns", "df64"],
+                       cwd=PKG, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     res = json.load(open(tmp_path / "summary" / "all_results.json"))
-    assert len(res) == 2 * 3 * 2
+    assert len(res) == 2 * 4 * 2
     for x in res:
         assert {"cmf", "variant", "mode", "rank", "N", "h", "n_traj", "gpu_time_ms",
                 "throughput_traj_per_s", "mem_used_mb", "n_valid", "n_with_limits",
