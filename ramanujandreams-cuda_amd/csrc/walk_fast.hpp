@@ -727,7 +727,7 @@ __global__ __launch_bounds__(256) void k_walk_seg_rows(SegArgsT<Ar> S) {
     }
 }
 
-template <int D, class Ar>
+template <int D, class Ar, int PP>
 __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
     const WalkArgs &A = S.w;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
     // not an integer point, or a product that vanished or overflowed: the
     // reference-order walk decides this trajectory
     if (!combine_segments<D, Ar>(A, t, store, (size_t)i, S.nseg, TableBounds{S.b}))
-        walk_exact_traj<D>(A, t);
+        walk_exact_traj<D, PP>(A, t);
 }
 
 // Row-split combine: G lanes per trajectory (G = D rounded up to a power of
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_walk_combine(SegArgsT<Ar> S) {
 // as in combine_segments, so the results are bit-identical.  For small
 // walks, where one lane per trajectory leaves the combine a long serial
 // chain of dependent loads and products on a handful of waves.
-template <int D, int G, class Ar>
+template <int D, int G, class Ar, int PP>
 __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
     using T = typename Ar::T;
     using Mg = typename Ar::M;
@@ -857,7 +857,7 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
     // not an integer point, or a product that vanished or overflowed: the
     // reference-order walk decides this trajectory
     if (exact) {
-        walk_exact_traj<D>(A, t);
+        walk_exact_traj<D, PP>(A, t);
         return;
     }
     if (A.n_fwd) A.n_fwd[t] = nf;
@@ -888,10 +888,10 @@ hipError_t launch_fast_pp(const SegArgsT<Ar> &S, int layout, hipStream_t st) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (layout == kSegTrajMajor)
-        hipLaunchKernelGGL((k_walk_combine_rows<D, G, Ar>),
+        hipLaunchKernelGGL((k_walk_combine_rows<D, G, Ar, PP>),
                            dim3((unsigned)(((size_t)S.w.n * G + bs - 1) / bs)), dim3(bs), 0, st, S);
     else
-        hipLaunchKernelGGL((k_walk_combine<D, Ar>), dim3(gx), dim3(bs), 0, st, S);
+        hipLaunchKernelGGL((k_walk_combine<D, Ar, PP>), dim3(gx), dim3(bs), 0, st, S);
     return hipGetLastError();
 }
 
