@@ -30,7 +30,18 @@ struct ArithF64 {
     __device__ static T mul(T a, T b) { return a * b; }
     __device__ static T fma(T a, T b, T c) { return ::fma(a, b, c); }
     __device__ static T neg(T a) { return -a; }
-    __device__ static T recip(T a) { return 1.0 / a; }
+    // 1/a from the hardware reciprocal estimate and two Newton steps
+    // r <- r + r (1 - a r) (quadratic convergence: the estimate's ~2^-23
+    // relative error falls below an ulp); within an ulp of the rounded
+    // quotient, without the scaling and fix-up steps of IEEE division.  For
+    // the walk's normal operands (lead, step and solve denominators) only.
+    __device__ static T recip(T a) {
+        double r = __builtin_amdgcn_rcp(a);
+        double e = ::fma(-a, r, 1.0);
+        r = ::fma(r, e, r);
+        e = ::fma(-a, r, 1.0);
+        return ::fma(r, e, r);
+    }
     __device__ static M mag(T a) { return fabs(a); }
     __device__ static T scale2(T a, int e) { return ldexp(a, e); }
     __device__ static bool finite(T a) { return isfinite(a); }

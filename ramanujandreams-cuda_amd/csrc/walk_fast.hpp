@@ -101,76 +101,61 @@ __device__ __forceinline__ bool load_fast(const WalkArgs &A, int t, int b0, Fast
     return ok;
 }
 
-// theta-companion column at the evaluation point of an axis step: x_j, and
-// y_j - 1, of the current position, the stepped axis shifted by one on an
-// inverse step (apply_axis_step, cuda/cmf_pfq.cuh:255-291).  The theta
-// polynomials are built with fused multiply-adds from the integer
-// coordinates, the vanishing test of the lead is the reference's
-// (cuda/cmf_pfq.cuh:146-181; the lead is exact for integer points).  `neg`
-// reports a negative evaluation coordinate.
-template <int D, class Ar, int PP = 0>
-__device__ __forceinline__ bool companion_column_fast(const FastPos &P, int ix, int iy, bool inverse,
-                                                      int p, int q, double z,
-                                                      typename Ar::T (&col)[D], bool &neg) {
+// theta polynomial prod_{j < n} (theta + c_j + bias) of integer coordinates,
+// monic, coefficients ascending, built with fused multiply-adds; coordinate
+// sj is shifted by sh (the evaluation point of an inverse step,
+// apply_axis_step, cuda/cmf_pfq.cuh:255-291).  sabs / neg receive the sum
+// of |e| and whether some e < 0 over the evaluated coordinates e.
+template <class Ar>
+__device__ __forceinline__ void theta_poly(const int32_t (&c)[8], int n, int bias, int sj, int sh,
+                                           typename Ar::T (&out)[9], int &sabs, bool &neg) {
     using T = typename Ar::T;
-    if (PP > 0) {  // PP F (PP-1) known at compile time
-        p = PP;
-        q = PP - 1;
-    }
-    T px[9], py[9];
+    out[0] = Ar::one();
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        px[k] = Ar::zero();
-        py[k] = Ar::zero();
-    }
-    px[0] = Ar::one();
-    py[0] = Ar::one();
-    int sabs = 0;
+    for (int k = 1; k < 9; ++k) out[k] = Ar::zero();
+    sabs = 0;
     neg = false;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        if (j < p) {
-            const int e = P.x[j] - ((inverse && j == ix) ? 1 : 0);
+        if (j < n) {
+            const int e = c[j] + bias + (j == sj ? sh : 0);
             sabs += abs(e);
             neg = neg || e < 0;
             const T v = Ar::from_d((double)e);
+            out[j + 1] = Ar::one();
 #pragma unroll
-            for (int k = j + 1; k >= 1; --k) px[k] = Ar::fma(px[k], v, px[k - 1]);
-            px[0] = Ar::mul(px[0], v);
-        }
-        if (j < q) {
-            const int e = P.y[j] - 1 + ((inverse && j == iy) ? 1 : 0);
-            sabs += abs(e);
-            neg = neg || e < 0;
-            const T v = Ar::from_d((double)e);
-#pragma unroll
-            for (int k = j + 1; k >= 1; --k) py[k] = Ar::fma(py[k], v, py[k - 1]);
-            py[0] = Ar::mul(py[0], v);
+            for (int k = j; k >= 1; --k) out[k] = Ar::fma(out[k], v, out[k - 1]);
+            out[0] = Ar::mul(out[0], v);
         }
     }
-    const T mz = Ar::from_d(-z);
-    T dc[10];
+}
+
+// Coefficients of D(theta) = theta prod(theta + y_j - 1) - z prod(theta + x_j)
+// from the two theta polynomials, the reference's vanishing test of the lead
+// (cuda/cmf_pfq.cuh:146-181; for an integer point the lead is exact in
+// either arithmetic) and nil = -1/lead.
+template <int D, class Ar>
+__device__ __forceinline__ bool theta_lead(const typename Ar::T (&px)[9], const typename Ar::T (&py)[9],
+                                           int p, int q, double z, int sabs, typename Ar::T (&dc)[10],
+                                           typename Ar::T &nil) {
+    const typename Ar::T mz = Ar::from_d(-z);
 #pragma unroll
-    for (int k = 0; k < 10; ++k) dc[k] = Ar::zero();
-#pragma unroll
-    for (int k = 0; k <= 8; ++k)
-        if (k <= p) dc[k] = Ar::mul(mz, px[k]);
-#pragma unroll
-    for (int k = 0; k <= 8; ++k)
-        if (k <= q) dc[k + 1] = Ar::add(dc[k + 1], py[k]);
-    // lead of D(theta) and the reference's vanishing test; for an integer
-    // point the lead is exact in either arithmetic
+    for (int k = 0; k < 10; ++k) {
+        const bool hx = k <= p, hy = k >= 1 && k - 1 <= q;
+        if (hx && hy)
+            dc[k] = Ar::fma(mz, px[k < 9 ? k : 8], py[k >= 1 ? k - 1 : 0]);
+        else if (hx)
+            dc[k] = Ar::mul(mz, px[k < 9 ? k : 8]);
+        else if (hy)
+            dc[k] = py[k >= 1 ? k - 1 : 0];
+        else
+            dc[k] = Ar::zero();
+    }
     const double lead = Ar::to_d(dc[D]);
     const double lscale = (D == p && p == q + 1) ? fmax(1.0, fabs(1.0 - z)) : 1.0 + (double)sabs;
     if (!isfinite(lead) || fabs(lead) <= 64.0 * kEps * lscale) return false;
-    const T nil = Ar::neg(Ar::recip(dc[D]));
-    bool ok = true;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-        col[i] = Ar::mul(dc[i], nil);
-        ok = ok && Ar::finite(col[i]);
-    }
-    return ok;
+    nil = Ar::neg(Ar::recip(dc[D]));
+    return true;
 }
 
 // Keeps max |M| inside [2^-R, 2^R] (R = Ar::kRangeLog2) with exact
@@ -234,11 +219,12 @@ __device__ __forceinline__ bool keep_in_range(typename Ar::T (&M)[D][D], int &e2
 }
 
 // The row-independent part of an axis step (apply_axis_step +
-// right_mul_affine_companion, cuda/cmf_pfq.cuh:186-291): the column, 1/a
-// and, for an inverse step, beta and 1/sd.  Returns false where the
-// reference fails (its decisions, see the file comment).  `tsc` receives a
-// bound of the step matrix's entries (max |M| times tsc bounds the terms of
-// the product's entries).
+// right_mul_affine_companion, cuda/cmf_pfq.cuh:186-291), from the theta
+// polynomials px, py at its evaluation point: for a forward step the
+// column over a (col / a) and 1/a; for an inverse step the column, 1/a,
+// beta and 1/sd.  Returns false where the reference fails (its decisions,
+// see the file comment).  `tsc` receives a bound of the step matrix's
+// entries (max |M| times tsc bounds the terms of the product's entries).
 template <int D, class Ar>
 struct StepCoef {
     typename Ar::T col[D], be[D], ia, isd;
@@ -246,8 +232,10 @@ struct StepCoef {
 };
 
 template <int D, class Ar, int PP = 0>
-__device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, double z, int ax,
-                                             int sign, StepCoef<D, Ar> &c, double &tsc) {
+__device__ __forceinline__ bool step_prepare_polys(const typename Ar::T (&px)[9],
+                                                   const typename Ar::T (&py)[9], int sabs, bool neg,
+                                                   const FastPos &P, int p, int q, double z, int ax,
+                                                   int sign, StepCoef<D, Ar> &c, double &tsc) {
     using T = typename Ar::T;
     if (PP > 0) {
         p = PP;
@@ -259,17 +247,33 @@ __device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, dou
     c.inverse = inverse;
     const int pos = isx ? pick8(P.x, ix) : pick8(P.y, iy);
     const int denom_i = isx ? (inverse ? pos - 1 : pos) : (inverse ? pos : pos - 1);
-    bool neg;
-    if (!companion_column_fast<D, Ar, PP>(P, ix, iy, inverse, p, q, z, c.col, neg)) return false;
+    T dc[10], nil;
+    if (!theta_lead<D, Ar>(px, py, p, q, z, sabs, dc, nil)) return false;
     if (denom_i == 0) return false;  // |denom| <= 64 eps for an integer point
     const double denom = (double)denom_i;
     c.ia = Ar::recip(Ar::from_d(denom));
+    const double aia = fabs(Ar::to_d(c.ia));
+    bool ok = true;
     double scol = 0.0;
+    if (!inverse) {
+        // M <- M (I + C/a): the column enters over a only
+        const T s = Ar::mul(nil, c.ia);
 #pragma unroll
-    for (int k = 0; k < D; ++k) scol += fabs(Ar::to_d(c.col[k]));
-    const double aia = fabs(1.0 / denom);
-    tsc = fmax(1.0, aia) * fmax(1.0, scol * aia);
-    if (!inverse) return true;
+        for (int i = 0; i < D; ++i) {
+            c.col[i] = Ar::mul(dc[i], s);
+            ok = ok && Ar::finite(c.col[i]);
+            scol += fabs(Ar::to_d(c.col[i]));
+        }
+        tsc = fmax(1.0, aia) * fmax(1.0, scol);
+        return ok;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        c.col[i] = Ar::mul(dc[i], nil);
+        ok = ok && Ar::finite(c.col[i]);
+        scol += fabs(Ar::to_d(c.col[i]));
+    }
+    if (!ok) return false;
     c.be[D - 1] = Ar::one();
 #pragma unroll
     for (int j = D - 2; j >= 0; --j) c.be[j] = Ar::mul(Ar::neg(c.be[j + 1]), c.ia);
@@ -320,17 +324,37 @@ __device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, dou
     return true;
 }
 
-// One row of M <- M * (I + C/a) or M * (I + C/a)^-1.
+// The same from the position alone (both theta polynomials built here).
+template <int D, class Ar, int PP = 0>
+__device__ __forceinline__ bool step_prepare(const FastPos &P, int p, int q, double z, int ax,
+                                             int sign, StepCoef<D, Ar> &c, double &tsc) {
+    using T = typename Ar::T;
+    if (PP > 0) {
+        p = PP;
+        q = PP - 1;
+    }
+    const bool isx = ax < p;
+    const bool inverse = isx ? (sign < 0) : (sign > 0);
+    T px[9], py[9];
+    int sx, sy;
+    bool nx, ny;
+    theta_poly<Ar>(P.x, p, 0, isx ? ax : -1, inverse ? -1 : 0, px, sx, nx);
+    theta_poly<Ar>(P.y, q, -1, isx ? -1 : ax - p, inverse ? 1 : 0, py, sy, ny);
+    return step_prepare_polys<D, Ar, PP>(px, py, sx + sy, nx || ny, P, p, q, z, ax, sign, c, tsc);
+}
+
+// One row of M <- M * (I + C/a) (c.col holds C's column over a) or
+// M * (I + C/a)^-1.
 template <int D, class Ar>
 __device__ __forceinline__ void step_apply_row(typename Ar::T (&m)[D], const StepCoef<D, Ar> &c) {
     using T = typename Ar::T;
     if (!c.inverse) {
-        T dot = Ar::mul(m[0], c.col[0]);
+        T dot = Ar::fma(m[0], c.col[0], m[D - 1]);
 #pragma unroll
         for (int k = 1; k < D; ++k) dot = Ar::fma(m[k], c.col[k], dot);
 #pragma unroll
         for (int j = 0; j + 1 < D; ++j) m[j] = Ar::fma(m[j + 1], c.ia, m[j]);
-        m[D - 1] = Ar::fma(dot, c.ia, m[D - 1]);
+        m[D - 1] = dot;
         return;
     }
     T al[D];
@@ -343,6 +367,107 @@ __device__ __forceinline__ void step_apply_row(typename Ar::T (&m)[D], const Ste
     const T tt = Ar::mul(Ar::fma(Ar::neg(da), c.ia, m[D - 1]), c.isd);
 #pragma unroll
     for (int j = 0; j < D; ++j) m[j] = Ar::fma(c.be[j], tt, al[j]);
+}
+
+// The wave-uniform loop is compiled in for ranks up to 6: at rank 7 and 8
+// the kernel holding both loops needs more than 256 registers per lane
+// (one resident wave per SIMD instead of two).
+template <int D>
+constexpr bool kUniformLoop = D <= 6;
+
+// Direction words of a walk: per lane, or one set for the whole wave
+// (scalar registers) when every lane walks the same direction.
+struct DirWords {
+    uint32_t w[4];
+    int top;
+};
+
+__device__ __forceinline__ int dir_of(const DirWords &d, int ax) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w |= d.w[k] & (0u - (uint32_t)(k == (ax >> 2)));
+    return (int)(int8_t)((w >> (8 * (ax & 3))) & 0xffu);
+}
+
+// The trajectory steps of walk_segment from the identity in M.  UNI: the
+// direction (du) is the same on every lane of the wave, so the axis
+// sequence, and with it every branch on the step kind, is wave-uniform;
+// lanes differ only in their segment length and in failing.
+template <int D, class Ar, int PP, bool UNI>
+__device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, FastPos &P,
+                                         const DirWords &du, typename Ar::T (&M)[D][D], int &e2,
+                                         int &nf, int &ni) {
+    const int wp = PP > 0 ? PP : A.p;
+    const int naxes = PP > 0 ? 2 * PP - 1 : A.naxes;
+    typename Ar::M mx = 1;  // max of the identity
+    // Range checks.  Where the collapse check applies (rank <= 4, first
+    // kCollapseSteps steps) the max is taken after every axis step.
+    // Elsewhere only the growth is bounded per axis step (the product's
+    // entries are sums of D terms, each at most max |M| times tsc) and the
+    // max is taken when that bound reaches Ar::kGrowBudget powers of two,
+    // and after every trajectory step (shrinkage, vanished or non-finite
+    // products).  The rescales are exact powers of two, so their timing
+    // does not change the product.
+    int grow = 0;
+    int res = 1;  // 1 walking, 0 failed, 2 the reference order decides
+    for (int st = 0;; ++st) {
+        const bool live0 = res == 1 && st < steps;
+        if (UNI ? !__any(live0) : !live0) break;
+        bool live = live0;
+        const bool watch = D <= kCollapseMaxRank && b0 + st < kCollapseSteps;
+#pragma unroll 1
+        for (int lev = du.top; lev >= 1; --lev) {
+#pragma unroll 1
+            for (int ax = naxes - 1; ax >= 0; --ax) {
+                const int dv = dir_of(du, ax);
+                if (abs(dv) < lev) continue;
+                const bool isx = ax < wp;
+                if (live) {
+                    const int sign = dv > 0 ? 1 : -1;
+                    StepCoef<D, Ar> c;
+                    double tsc;
+                    const bool ok = step_prepare<D, Ar, PP>(P, A.p, A.q, A.z, ax, sign, c, tsc);
+                    const bool inverse = c.inverse;
+                    nf += inverse ? 0 : 1;
+                    ni += inverse ? 1 : 0;
+                    if (!ok) {
+                        res = 0;
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < D; ++r) step_apply_row<D, Ar>(M[r], c);
+                        if (watch) {
+                            if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                                res = 2;  // product vanished or overflowed
+                        } else {
+                            // tsc >= 1; an infinite or NaN bound forces a check
+                            const int g = (tsc < 0x1p1000) ? ilogb(tsc) + 1 : 1000;
+                            grow += g + 3;
+                            if (grow > Ar::kGrowBudget) {
+                                if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) res = 2;
+                                grow = 0;
+                            }
+                        }
+                    }
+                    live = res == 1;
+                    if (live) {
+                        // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            P.x[j] += (isx && j == ax) ? sign : 0;
+                            P.y[j] += (!isx && j == ax - wp) ? sign : 0;
+                        }
+                    }
+                }
+            }
+        }
+        if (live && !watch && grow > 0) {
+            if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) res = 2;
+            grow = 0;
+        }
+    }
+    if (res != 1) return res;
+    if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) return 2;
+    return 1;
 }
 
 // One axis step M <- M * (I + C/a)^(+-1) at position P (which the caller
@@ -366,12 +491,13 @@ __device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const 
 // grid, or a product that vanished or overflowed).  nf / ni count the
 // executed forward / inverse axis steps, the failing one included.
 // PP > 0: the walk is of PP F (PP-1) (A.p, A.q equal to that), known at
-// compile time.
+// compile time.  Lanes of a wave walking one direction (direction-sorted
+// batches, a wave on the segments of one trajectory) take the wave-uniform
+// loop; its arithmetic is the per-lane loop's, so the result does not
+// depend on the lane's neighbours.
 template <int D, class Ar, int PP = 0>
 __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typename Ar::T (&M)[D][D],
                             int &e2, int &nf, int &ni) {
-    const int wp = PP > 0 ? PP : A.p;
-    const int naxes = PP > 0 ? 2 * PP - 1 : A.naxes;
     FastPos P;
     int top;
     nf = ni = e2 = 0;
@@ -380,59 +506,21 @@ __device__ int walk_segment(const WalkArgs &A, int t, int b0, int steps, typenam
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) M[i][j] = (i == j) ? Ar::one() : Ar::zero();
-    typename Ar::M mx = 1;  // max of the identity
-    // Range checks.  Where the collapse check applies (rank <= 4, first
-    // kCollapseSteps steps) the max is taken after every axis step.
-    // Elsewhere only the growth is bounded per axis step (the product's
-    // entries are sums of D terms, each at most max |M| times tsc) and the
-    // max is taken when that bound reaches Ar::kGrowBudget powers of two,
-    // and after every trajectory step (shrinkage, vanished or non-finite
-    // products).  The rescales are exact powers of two, so their timing
-    // does not change the product.
-    int grow = 0;
-    bool ok = true;
-    for (int st = 0; ok && st < steps; ++st) {
-        const bool watch = D <= kCollapseMaxRank && b0 + st < kCollapseSteps;
-        for (int lev = top; ok && lev >= 1; --lev) {
-            for (int ax = naxes - 1; ax >= 0; --ax) {
-                const int dv = dir_at(P, ax);
-                if (abs(dv) < lev) continue;
-                const int sign = dv > 0 ? 1 : -1;
-                bool inverse;
-                double tsc;
-                ok = axis_step_fast<D, Ar, PP>(M, P, A.p, A.q, A.z, ax, sign, inverse, tsc);
-                nf += inverse ? 0 : 1;
-                ni += inverse ? 1 : 0;
-                if (!ok) break;
-                if (watch) {
-                    if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
-                        return 2;  // product vanished or overflowed
-                } else {
-                    // tsc >= 1; an infinite or NaN bound forces a check
-                    const int g = (tsc < 0x1p1000) ? ilogb(tsc) + 1 : 1000;
-                    grow += g + 3;
-                    if (grow > Ar::kGrowBudget) {
-                        if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) return 2;
-                        grow = 0;
-                    }
-                }
-                // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
-                const bool isx = ax < wp;
+    DirWords du;
+    bool same = true;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    P.x[j] += (isx && j == ax) ? sign : 0;
-                    P.y[j] += (!isx && j == ax - wp) ? sign : 0;
-                }
-            }
-        }
-        if (ok && !watch && grow > 0) {
-            if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) return 2;
-            grow = 0;
-        }
+    for (int k = 0; k < 4; ++k) {
+        du.w[k] = __builtin_amdgcn_readfirstlane(P.dpk[k]);
+        same = same && du.w[k] == P.dpk[k];
     }
-    if (!ok) return 0;
-    if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) return 2;
-    return 1;
+    du.top = __builtin_amdgcn_readfirstlane(top);
+    same = same && du.top == top;
+    if (kUniformLoop<D> && __all(same))
+        return walk_loop<D, Ar, PP, true>(A, b0, steps, P, du, M, e2, nf, ni);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) du.w[k] = P.dpk[k];
+    du.top = top;
+    return walk_loop<D, Ar, PP, false>(A, b0, steps, P, du, M, e2, nf, ni);
 }
 
 // Segment products and their bookkeeping, slot-major: entry (s, slot) at
