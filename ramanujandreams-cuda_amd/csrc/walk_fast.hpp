@@ -389,6 +389,13 @@ __device__ __forceinline__ int dir_of(const DirWords &d, int ax) {
     return (int)(int8_t)((w >> (8 * (ax & 3))) & 0xffu);
 }
 
+// Axes with |direction| >= lev, one bit each.
+__device__ __forceinline__ uint32_t level_mask(const DirWords &d, int naxes, int lev) {
+    uint32_t m = 0;
+    for (int ax = 0; ax < naxes; ++ax) m |= (uint32_t)(abs(dir_of(d, ax)) >= lev) << ax;
+    return m;
+}
+
 // The trajectory steps of walk_segment from the identity in M.  UNI: the
 // direction (du) is the same on every lane of the wave, so the axis
 // sequence, and with it every branch on the step kind, is wave-uniform;
@@ -410,6 +417,10 @@ __device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, F
     // does not change the product.
     int grow = 0;
     int res = 1;  // 1 walking, 0 failed, 2 the reference order decides
+    // wave-uniform walks: the axes moving at levels 1..4, 16 bits each
+    uint64_t lm = 0;
+    if (UNI)
+        for (int l = 1; l <= 4 && l <= du.top; ++l) lm |= (uint64_t)level_mask(du, naxes, l) << (16 * (l - 1));
     for (int st = 0;; ++st) {
         const bool live0 = res == 1 && st < steps;
         if (UNI ? !__any(live0) : !live0) break;
@@ -417,45 +428,60 @@ __device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, F
         const bool watch = D <= kCollapseMaxRank && b0 + st < kCollapseSteps;
 #pragma unroll 1
         for (int lev = du.top; lev >= 1; --lev) {
+            // UNI: the axes moving at this level as a scalar bit mask,
+            // highest first (axes that do not move cost nothing)
+            uint32_t m = 0;
+            if (UNI) m = lev <= 4 ? (uint32_t)((lm >> (16 * (lev - 1))) & 0xffffu) : level_mask(du, naxes, lev);
+            int an = naxes;
 #pragma unroll 1
-            for (int ax = naxes - 1; ax >= 0; --ax) {
-                const int dv = dir_of(du, ax);
-                if (abs(dv) < lev) continue;
+            for (;;) {
+                int ax, dv;
+                if (UNI) {
+                    if (!m) break;
+                    ax = 31 - __builtin_clz(m);
+                    m ^= 1u << ax;
+                    dv = dir_of(du, ax);
+                } else {
+                    if (--an < 0) break;
+                    ax = an;
+                    dv = dir_of(du, ax);
+                    if (abs(dv) < lev) continue;
+                }
+                // one axis step of sign(dv) on axis ax (wave-uniform), for live lanes
                 const bool isx = ax < wp;
-                if (live) {
-                    const int sign = dv > 0 ? 1 : -1;
-                    StepCoef<D, Ar> c;
-                    double tsc;
-                    const bool ok = step_prepare<D, Ar, PP>(P, A.p, A.q, A.z, ax, sign, c, tsc);
-                    const bool inverse = c.inverse;
-                    nf += inverse ? 0 : 1;
-                    ni += inverse ? 1 : 0;
-                    if (!ok) {
-                        res = 0;
-                    } else {
+                if (!live) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                StepCoef<D, Ar> c;
+                double tsc;
+                const bool ok = step_prepare<D, Ar, PP>(P, A.p, A.q, A.z, ax, sign, c, tsc);
+                const bool inverse = c.inverse;
+                nf += inverse ? 0 : 1;
+                ni += inverse ? 1 : 0;
+                if (!ok) {
+                    res = 0;
+                } else {
 #pragma unroll
-                        for (int r = 0; r < D; ++r) step_apply_row<D, Ar>(M[r], c);
-                        if (watch) {
-                            if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
-                                res = 2;  // product vanished or overflowed
-                        } else {
-                            // tsc >= 1; an infinite or NaN bound forces a check
-                            const int g = (tsc < 0x1p1000) ? ilogb(tsc) + 1 : 1000;
-                            grow += g + 3;
-                            if (grow > Ar::kGrowBudget) {
-                                if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) res = 2;
-                                grow = 0;
-                            }
+                    for (int r = 0; r < D; ++r) step_apply_row<D, Ar>(M[r], c);
+                    if (watch) {
+                        if (!keep_in_range<D, Ar>(M, e2, false, (double)mx * tsc, A.collapse, mx))
+                            res = 2;  // product vanished or overflowed
+                    } else {
+                        // tsc >= 1; an infinite or NaN bound forces a check
+                        const int g = (tsc < 0x1p1000) ? ilogb(tsc) + 1 : 1000;
+                        grow += g + 3;
+                        if (grow > Ar::kGrowBudget) {
+                            if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) res = 2;
+                            grow = 0;
                         }
                     }
-                    live = res == 1;
-                    if (live) {
-                        // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
+                }
+                live = res == 1;
+                if (live) {
+                    // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            P.x[j] += (isx && j == ax) ? sign : 0;
-                            P.y[j] += (!isx && j == ax - wp) ? sign : 0;
-                        }
+                    for (int j = 0; j < 8; ++j) {
+                        P.x[j] += (isx && j == ax) ? sign : 0;
+                        P.y[j] += (!isx && j == ax - wp) ? sign : 0;
                     }
                 }
             }
@@ -468,21 +494,6 @@ __device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, F
     if (res != 1) return res;
     if (!keep_in_range<D, Ar>(M, e2, true, 0.0, A.collapse, mx)) return 2;
     return 1;
-}
-
-// One axis step M <- M * (I + C/a)^(+-1) at position P (which the caller
-// moves afterwards).
-template <int D, class Ar, int PP = 0>
-__device__ __forceinline__ bool axis_step_fast(typename Ar::T (&M)[D][D], const FastPos &P, int p,
-                                               int q, double z, int ax, int sign, bool &inverse,
-                                               double &tsc) {
-    StepCoef<D, Ar> c;
-    const bool ok = step_prepare<D, Ar, PP>(P, p, q, z, ax, sign, c, tsc);
-    inverse = c.inverse;
-    if (!ok) return false;
-#pragma unroll
-    for (int r = 0; r < D; ++r) step_apply_row<D, Ar>(M[r], c);
-    return true;
 }
 
 // Walks trajectory steps (b0, b0 + steps] of trajectory t from the identity.
