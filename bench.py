@@ -207,6 +207,78 @@ def timed_steps(torch, fn, steps, warmup, after_warmup=None):
     return out, (time.perf_counter() - t0) / steps, T
 
 
+def plain_6f5(torch, cw, np, reps=3):
+    """The plain 6F5 f64 walks beside the headline search.
+
+    fp64_walk: 6F5, z=1/2, N=2000, h=100, 65,536 trajectories, integer starts
+    uniform in {1..16} (mt19937_64 seed 42), x-directions in {0,1} and no
+    y-moves: the large compute-bound 6F5 walk (with all-ones directions no
+    6F5 walk of this shape stays valid to N=2000).  Canonical flops of the
+    executed axis steps over the walk's time (HIP events on the stream the
+    kernels run on) and over the segment kernel's own time, for the fast and
+    the reference-order engines.
+    c2: 6F5, z=1, zeta start, directions in {0..3}^11, 10,000 trajectories,
+    N=1000, h=50, walked f64 (reference order and fast) and RNS (8 moduli)."""
+    p, q = 6, 5
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    n, N, h, z = 65536, 2000, 100, 0.5
+    D, S = cw.synth_inputs(42, n, p, q, 1, 16, 0, 1)
+    D[:, p:p + q] = 0
+    D[D[:, :p].sum(1) == 0, 0] = 1
+    dd, ss = cw.to_device_inputs(D[:, :p + q], S[:, :p + q])
+    ff, fi = canonical_flops(cw.make_desc(p, q, z).rank, p, q)
+    walk = {}
+    for eng in ("walk_f64_fast", "walk_f64"):
+        fn = getattr(cw, eng)
+        out = cw.alloc_results(n)
+        ms = timed(lambda: fn(p, q, z, dd, ss, N, h, out=out))
+        r = cw.results_to_numpy(out)
+        fl = float(r["n_fwd"].astype(np.float64).sum() * ff + r["n_inv"].astype(np.float64).sum() * fi)
+        walk[eng] = {"ms": ms, "traj_per_s": n / (ms * 1e-3), "valid": int(r["valid"].sum()),
+                     "canonical_flops": fl, "tflops": fl / (ms * 1e-3) / 1e12,
+                     "frac_fp64_peak": fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+        if eng == "walk_f64_fast":
+            cw.kernel_time_reset()
+            cw.set_tuning("kernel_events", 1)
+            for _ in range(reps):
+                fn(p, q, z, dd, ss, N, h, out=out)
+            cw.set_tuning("kernel_events", 0)
+            kms, kn = cw.kernel_time("walk_seg_f64")
+            seg = kms / max(kn, 1)
+            walk[eng]["segment_kernel"] = {
+                "kernel": "k_walk_seg<6,f64>", "avg_launch_ms": seg,
+                "tflops": fl / (seg * 1e-3) / 1e12,
+                "frac_fp64_peak": fl / (seg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    n, N, h, z = 10000, 1000, 50, 1.0
+    D, S = cw.synth_inputs(42, n, p, q, 2, 1, 0, 3)
+    dd, ss = cw.to_device_inputs(D[:, :p + q], S[:, :p + q])
+    c2 = {}
+    for eng in ("walk_f64", "walk_f64_fast", "walk_rns"):
+        fn = getattr(cw, eng)
+        out = cw.alloc_results(n)
+        ms = timed(lambda: fn(p, q, z, dd, ss, N, h, out=out))
+        c2[eng] = {"ms": ms, "traj_per_s": n / (ms * 1e-3),
+                   "valid": int(cw.results_to_numpy(out)["valid"].sum())}
+    return {"fp64_walk": {"workload": "6F5 z=0.5 N=2000 h=100, 65536 trajectories, starts {1..16}, "
+                                      "x-directions {0,1}, no y-moves (mt19937_64 seed 42)",
+                          **walk},
+            "c2": {"workload": "6F5 z=1 N=1000 h=50, zeta start, directions {0..3}^11, "
+                               "10000 trajectories (mt19937_64 seed 42); RNS 8 moduli", **c2}}
+
+
 def load_traffic():
     """HBM bytes per launch of the walk kernels, from the rocprofv3 FETCH_SIZE
     and WRITE_SIZE passes summarised in profiles/hbm_traffic.json (already
@@ -474,6 +546,8 @@ def main():
                      "mse_last_epoch": hist[-1]},
         "roofline": roofline,
     }
+    log("plain 6F5 walks ...")
+    line["plain_f64_6f5"] = plain_6f5(torch, cw, np)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(srch.host_D, srch.host_S, args.cpu_sample,

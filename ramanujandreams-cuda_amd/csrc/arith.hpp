@@ -22,6 +22,11 @@ struct ArithF64 {
     // growth (log2 bound) allowed between two range checks: 2^(400 + 560)
     // stays below the largest double
     static constexpr int kGrowBudget = 560;
+    // the max is also taken at the end of a trajectory step once the
+    // growth count since the last check reaches this (each axis step adds at
+    // least 4): shrinkage of the product is caught within a few steps, with
+    // 2^622 of headroom below the window before doubles lose precision
+    static constexpr int kShrinkCheck = 32;
     __device__ static T zero() { return 0.0; }
     __device__ static T one() { return 1.0; }
     __device__ static T from_d(double x) { return x; }
@@ -58,6 +63,7 @@ struct ArithFF {
     static constexpr const char *kSegKernel = "walk_seg_f32x2";  // kernel timing key
     static constexpr int kRangeLog2 = 60;  // keep max |hi| in [2^-60, 2^60]
     static constexpr int kGrowBudget = 56;  // 2^(60 + 56) below the largest float
+    static constexpr int kShrinkCheck = 1;  // after every trajectory step
     __device__ static T zero() { return {0.f, 0.f}; }
     __device__ static T one() { return {1.f, 0.f}; }
     __device__ static T from_d(double x) {

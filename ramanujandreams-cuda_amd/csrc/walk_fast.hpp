@@ -389,6 +389,22 @@ __device__ __forceinline__ int dir_of(const DirWords &d, int ax) {
     return (int)(int8_t)((w >> (8 * (ax & 3))) & 0xffu);
 }
 
+// Moves coordinate j of x (isx) or y by sign; j, isx and sign are
+// wave-uniform, so a scalar branch picks the one register to update.
+__device__ __forceinline__ void move_uniform(FastPos &P, int j, bool isx, int sign) {
+    int32_t(&c)[8] = isx ? P.x : P.y;
+    switch (j) {
+        case 0: c[0] += sign; break;
+        case 1: c[1] += sign; break;
+        case 2: c[2] += sign; break;
+        case 3: c[3] += sign; break;
+        case 4: c[4] += sign; break;
+        case 5: c[5] += sign; break;
+        case 6: c[6] += sign; break;
+        default: c[7] += sign; break;
+    }
+}
+
 // Axes with |direction| >= lev, one bit each.
 __device__ __forceinline__ uint32_t level_mask(const DirWords &d, int naxes, int lev) {
     uint32_t m = 0;
@@ -412,9 +428,10 @@ __device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, F
     // Elsewhere only the growth is bounded per axis step (the product's
     // entries are sums of D terms, each at most max |M| times tsc) and the
     // max is taken when that bound reaches Ar::kGrowBudget powers of two,
-    // and after every trajectory step (shrinkage, vanished or non-finite
-    // products).  The rescales are exact powers of two, so their timing
-    // does not change the product.
+    // and at the end of a trajectory step once the count reaches
+    // Ar::kShrinkCheck (shrinkage, vanished or non-finite products).  The
+    // rescales are exact powers of two, so their timing does not change the
+    // product.
     int grow = 0;
     int res = 1;  // 1 walking, 0 failed, 2 the reference order decides
     // wave-uniform walks: the axes moving at levels 1..4, 16 bits each
@@ -478,15 +495,19 @@ __device__ __forceinline__ int walk_loop(const WalkArgs &A, int b0, int steps, F
                 live = res == 1;
                 if (live) {
                     // move the position (apply_axis_step, cuda/cmf_pfq.cuh:292-298)
+                    if (UNI) {
+                        move_uniform(P, isx ? ax : ax - wp, isx, sign);
+                    } else {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        P.x[j] += (isx && j == ax) ? sign : 0;
-                        P.y[j] += (!isx && j == ax - wp) ? sign : 0;
+                        for (int j = 0; j < 8; ++j) {
+                            P.x[j] += (isx && j == ax) ? sign : 0;
+                            P.y[j] += (!isx && j == ax - wp) ? sign : 0;
+                        }
                     }
                 }
             }
         }
-        if (live && !watch && grow > 0) {
+        if (live && !watch && grow >= Ar::kShrinkCheck) {
             if (!keep_in_range<D, Ar>(M, e2, false, 0.0, 0.f, mx)) res = 2;
             grow = 0;
         }
