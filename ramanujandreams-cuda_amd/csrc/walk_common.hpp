@@ -135,6 +135,10 @@ struct LimitTally {
 };
 
 template <int D>
+__device__ void finish_tail(const WalkArgs &A, int t, double limit, int nlim, double p1, double q1,
+                            double p2, double q2, double p3, double q3);
+
+template <int D>
 __device__ void finish_from_snapshots(const WalkArgs &A, int t, bool ok) {
 #pragma clang fp contract(off)
     if (!ok) {
@@ -201,6 +205,17 @@ __device__ void finish_from_snapshots(const WalkArgs &A, int t, bool ok) {
     }
     const double nan = __builtin_nan("");
     double limit = T.has_aitken ? T.first_aitken : (T.n > 0 ? T.first : nan);
+    finish_tail<D>(A, t, limit, T.n, p1, q1, p2, q2, p3, q3);
+}
+
+// precision, delta and the result fields of a valid trajectory from its
+// limit and candidate count (compute_precision, compute_kamidelta,
+// cuda/cmf_pfq.cuh:531-596).
+template <int D>
+__device__ void finish_tail(const WalkArgs &A, int t, double limit, int nlim, double p1, double q1,
+                            double p2, double q2, double p3, double q3) {
+#pragma clang fp contract(off)
+    const double nan = __builtin_nan("");
     const double ls2 = A.snap_ls[1 * A.sld + t];
     const double ls3 = A.snap_ls[2 * A.sld + t];
 
@@ -263,13 +278,138 @@ __device__ void finish_from_snapshots(const WalkArgs &A, int t, bool ok) {
         }
     }
     if (A.valid) A.valid[t] = 1;
-    if (A.n_limits) A.n_limits[t] = T.n;
+    if (A.n_limits) A.n_limits[t] = nlim;
     if (A.limit) A.limit[t] = limit;
     if (A.rp) A.rp[t] = p3;
     if (A.rq) A.rq[t] = q3;
     if (A.log_scale) A.log_scale[t] = ls3;
     if (A.precision) A.precision[t] = prec;
     if (A.delta) A.delta[t] = delta;
+}
+
+// The same reductions with the NL lanes of a group (NL a power of two, at
+// most 64, the lanes [base, base + NL) of one wave, all active, each with
+// the same t and ok) splitting the candidate scan: candidate e = (c*D + i)*D
+// + j goes to lane e mod NL, each pair is evaluated by the operations of
+// finish_from_snapshots, and the scan order is restored by reductions: the
+// candidate count is a sum, the first Aitken candidate (kept only while
+// fewer than 256 candidates precede it) and the first candidate are the
+// smallest e that produced one.  Results are bit-identical to
+// finish_from_snapshots; lane sub == 0 writes them.  ceil(D^3 / NL) <= 64.
+template <int D>
+__device__ void finish_group(const WalkArgs &A, int t, bool ok, int sub, int NL) {
+#pragma clang fp contract(off)
+    if (!ok || D * D * D > 64 * NL) {  // nothing to scan, or too few lanes for the bit masks
+        if (sub == 0) finish_from_snapshots<D>(A, t, ok);
+        return;
+    }
+    const int base = (int)(threadIdx.x & 63) - sub;
+    const double DEN_MIN = 1e-280;
+    const int lc = D - 1;
+    const double p3 = snap_at(A, D, 2, 0, lc, t), q3 = snap_at(A, D, 2, 1, lc, t);
+    const double p2 = snap_at(A, D, 1, 0, lc, t), q2 = snap_at(A, D, 1, 1, lc, t);
+    const double p1 = snap_at(A, D, 0, 0, lc, t), q1 = snap_at(A, D, 0, 1, lc, t);
+    // the primary projection, on every lane
+    int pn = 0;
+    bool pa = false;
+    double pav = 0.0, pfirst = 0.0;
+    if (fabs(q3) > DEN_MIN && fabs(q2) > DEN_MIN && fabs(q1) > DEN_MIN) {
+        double r1 = p1 / q1, r2 = p2 / q2, r3 = p3 / q3;
+        if (isfinite(r1) && isfinite(r2) && isfinite(r3)) {
+            double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+            double den = r1 - 2.0 * r2 + r3;
+            if (fabs(den) > 1e-14 * sc) {
+                double dd = r2 - r3;
+                double L = r3 - dd * dd / den;
+                if (isfinite(L)) {
+                    pa = true;
+                    pav = L;
+                    pfirst = L;
+                    ++pn;
+                }
+            }
+            if (fabs(r3) > 1e-15) {
+                if (pn == 0) pfirst = r3;
+                ++pn;
+            }
+        }
+    }
+    // this lane's share of the scan, in increasing e
+    constexpr int kE = D * D * D;
+    uint64_t one = 0, two = 0;  // per iteration k: >= 1 candidate, 2 candidates
+    int ea = kE, ef = kE;       // this lane's first Aitken / first candidate
+    double av = 0.0, fv = 0.0;
+    int tot = 0;
+    for (int k = 0, e = sub; e < kE; ++k, e += NL) {
+        const int c = e / (D * D), i = (e / D) % D, j = e % D;
+        if (i == j || (i == 0 && j == 1 && c == D - 1)) continue;
+        const double d1 = snap_at(A, D, 0, j, c, t), d2 = snap_at(A, D, 1, j, c, t),
+                     d3 = snap_at(A, D, 2, j, c, t);
+        if (fabs(d1) < DEN_MIN || fabs(d2) < DEN_MIN || fabs(d3) < DEN_MIN) continue;
+        const double r1 = snap_at(A, D, 0, i, c, t) / d1, r2 = snap_at(A, D, 1, i, c, t) / d2,
+                     r3 = snap_at(A, D, 2, i, c, t) / d3;
+        if (!isfinite(r1) || !isfinite(r2) || !isfinite(r3)) continue;
+        double sc = fmax(1.0, fmax(fabs(r1), fmax(fabs(r2), fabs(r3))));
+        double den = r1 - 2.0 * r2 + r3;
+        int n = 0;
+        if (fabs(den) > 1e-14 * sc) {
+            double dd = r2 - r3;
+            double L = r3 - dd * dd / den;
+            if (isfinite(L) && fabs(L) > 1e-15) {
+                if (ea == kE) {
+                    ea = e;
+                    av = L;
+                }
+                if (ef == kE) {
+                    ef = e;
+                    fv = L;
+                }
+                ++n;
+            }
+        }
+        if (fabs(r3) > 1e-15) {
+            if (ef == kE) {
+                ef = e;
+                fv = r3;
+            }
+            ++n;
+        }
+        one |= (uint64_t)(n >= 1) << k;
+        two |= (uint64_t)(n == 2) << k;
+        tot += n;
+    }
+    // group reductions (xor partners stay inside the aligned group)
+    int Ea = ea, Ef = ef, Tot = tot;
+    for (int off = 1; off < NL; off <<= 1) {
+        Ea = min(Ea, __shfl_xor(Ea, off, 64));
+        Ef = min(Ef, __shfl_xor(Ef, off, 64));
+        Tot += __shfl_xor(Tot, off, 64);
+    }
+    // candidates of the scan before e = Ea
+    int before = 0;
+    if (Ea < kE && Ea > sub) {
+        const int kmax = (Ea - sub + NL - 1) / NL;  // iterations with e < Ea
+        const uint64_t mk = kmax >= 64 ? ~0ull : ((1ull << kmax) - 1);
+        before = __popcll(one & mk) + __popcll(two & mk);
+    }
+    for (int off = 1; off < NL; off <<= 1) before += __shfl_xor(before, off, 64);
+    const double avE = __shfl(av, base + (Ea < kE ? Ea % NL : 0), 64);
+    const double fvE = __shfl(fv, base + (Ef < kE ? Ef % NL : 0), 64);
+    if (sub != 0) return;
+    const double nan = __builtin_nan("");
+    double limit;
+    if (pa)
+        limit = pav;
+    else if (Ea < kE && pn + before < 256)
+        limit = avE;
+    else if (pn > 0)
+        limit = pfirst;
+    else if (Ef < kE)
+        limit = fvE;
+    else
+        limit = nan;
+    const int nlim = min(256, pn + Tot);
+    finish_tail<D>(A, t, limit, nlim, p1, q1, p2, q2, p3, q3);
 }
 
 // --------------------------------------------------------------------------

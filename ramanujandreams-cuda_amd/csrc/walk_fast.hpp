@@ -973,18 +973,19 @@ __global__ __launch_bounds__(256) void k_walk_combine_rows(SegArgsT<Ar> S) {
         si = inn;
         se = en;
     }
-    if (r != 0) return;
     // not an integer point, or a product that vanished or overflowed: the
     // reference-order walk decides this trajectory
     if (exact) {
-        walk_exact_traj<D, PP>(A, t);
+        if (r == 0) walk_exact_traj<D, PP>(A, t);
         return;
     }
-    if (A.n_fwd) A.n_fwd[t] = nf;
-    if (A.n_inv) A.n_inv[t] = ni;
-    // the group's snapshot rows are read back by lane 0 of the same wave
+    if (r == 0) {
+        if (A.n_fwd) A.n_fwd[t] = nf;
+        if (A.n_inv) A.n_inv[t] = ni;
+    }
+    // the group's snapshot rows are read back by the lanes of the same wave
     __threadfence_block();
-    finish_from_snapshots<D>(A, t, ok);
+    finish_group<D>(A, t, ok, r, G);
 }
 
 template <int D, class Ar, int PP>

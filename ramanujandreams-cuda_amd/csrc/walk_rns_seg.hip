@@ -548,10 +548,10 @@ __global__ __launch_bounds__(256) void k_rns_combine_rows(RnsSegArgs RA) {
     if (sub == 0) {
         if (A.n_fwd) A.n_fwd[t] = nf;
         if (A.n_inv) A.n_inv[t] = ni;
-        // the snapshot rows were written by the other lanes of this wave
-        __threadfence_block();
-        finish_from_snapshots<D>(A, t, ok);
     }
+    // the snapshot rows were written by the other lanes of this wave
+    __threadfence_block();
+    finish_group<D>(A, t, ok, sub, L);
 }
 
 template <int D>
