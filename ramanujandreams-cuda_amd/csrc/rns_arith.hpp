@@ -30,8 +30,14 @@ __device__ __forceinline__ uint32_t redc(uint64_t T, const MontMod &M) {
     r = (r >= M.m) ? r - M.m : r;
     return r;
 }
+// Product of two reduced residues a, b < m: T < m^2, so the reduction's
+// (T + q m) / 2^32 < m^2 / 2^32 + m < 1.25 m (m < 2^30) and one conditional
+// subtraction leaves it in [0, m).
 __device__ __forceinline__ uint32_t mmul(uint32_t a, uint32_t b, const MontMod &M) {
-    return redc((uint64_t)a * b, M);
+    const uint64_t T = (uint64_t)a * b;
+    const uint32_t q = (uint32_t)T * M.mp;
+    const uint32_t r = (uint32_t)((T + (uint64_t)q * M.m) >> 32);
+    return (r >= M.m) ? r - M.m : r;
 }
 // a*b + c (all Montgomery): c enters as c*R so one reduction suffices
 __device__ __forceinline__ uint32_t mfma(uint32_t a, uint32_t b, uint32_t c, const MontMod &M) {
