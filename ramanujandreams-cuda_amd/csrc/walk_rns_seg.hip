@@ -297,6 +297,79 @@ __global__ __launch_bounds__(256) void k_rns_reduce(RnsSegArgs RA) {
         for (int j = 0; j < D; ++j) RA.segP[((size_t)s0 * D * D + i * D + j) * pst + col] = Q[i][j];
 }
 
+// The same fold with a lane per (matrix row, modulus): 8G lanes per
+// (interval, trajectory), lane (r, k) carries row r of the interval's
+// product (D^2 modular products per segment instead of D^3); rows r >= D
+// shadow row 0 and write nothing.  Row r of P_s0 P_s0+1 ... is computed by
+// the products k_rns_reduce takes for that row, so the residues are
+// identical.  For G <= 8.
+template <int D>
+__global__ __launch_bounds__(256) void k_rns_reduce_rows(RnsSegArgs RA) {
+    const WalkArgs &A = RA.w;
+    const int G = RA.G;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = gtid % G;
+    const int r = (gtid / G) % 8;
+    const int item = gtid / (8 * G);
+    const int c = item / A.n, t = item % A.n;
+    if (c >= RA.ncrt) return;
+    // segments [s0, s1] of interval c
+    int s0 = 0, s1 = RA.nseg - 1, cnt = 0;
+    for (int s = 0; s < RA.nseg; ++s) {
+        if (RA.crtp[s + 1]) {
+            if (cnt == c) {
+                s1 = s;
+                break;
+            }
+            ++cnt;
+            s0 = s + 1;
+        }
+    }
+    if (s1 <= s0) return;
+    const size_t o0 = (size_t)s0 * A.n + t;
+    if (RA.segV[o0] == 2) return;  // off the integer grid: the sequential kernel walks it
+    const bool active = k < RA.np;
+    const int rr = r < D ? r : 0;
+    const MontMod M = RA.C.mods[active ? k : 0];
+    int nf = RA.segF[o0], ni = RA.segI[o0];
+    bool ok = RA.segV[o0] == 1;
+    const size_t pst = (size_t)A.n * RA.np;
+    const size_t col = (size_t)t * RA.np + (active ? k : 0);
+    uint32_t q[D];
+    if (ok) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) q[j] = RA.segP[((size_t)s0 * D * D + rr * D + j) * pst + col];
+    }
+    for (int s = s0 + 1; ok && s <= s1; ++s) {
+        const size_t o = (size_t)s * A.n + t;
+        nf += RA.segF[o];
+        ni += RA.segI[o];
+        if (RA.segV[o] != 1) {
+            ok = false;
+            break;
+        }
+        const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + col;
+        uint32_t row[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            uint64_t acc = 0;
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)q[kk] * P[(size_t)(kk * D + j) * pst];
+            row[j] = redc(acc, M);
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) q[j] = row[j];
+    }
+    if (k == 0 && r == 0) {
+        RA.segV[o0] = ok ? 1 : 0;
+        RA.segF[o0] = nf;
+        RA.segI[o0] = ni;
+    }
+    if (!ok || !active || r >= D) return;
+#pragma unroll
+    for (int j = 0; j < D; ++j) RA.segP[((size_t)s0 * D * D + r * D + j) * pst + col] = q[j];
+}
+
 // Garner CRT of one entry on moduli 0 and 1, with the reference's uint64
 // wrap-around (cuda/cmf_walk_rns.cu:232-244).
 __device__ __forceinline__ double crt_garner(uint32_t r0, uint32_t r1, int np, uint64_t m0inv,
@@ -571,9 +644,14 @@ static hipError_t launch_rns_seg_d(const RnsSegArgs &RA, hipStream_t st) {
         else
             hipLaunchKernelGGL((k_rns_segment<D, 0>), grid, dim3(bs), 0, st, RA);
         kernel_timer_end("rns_segment", t0, st);
-        if (RA.reduced)
-            hipLaunchKernelGGL(k_rns_reduce<D>, dim3((int)((threads * RA.ncrt + bs - 1) / bs)),
-                               dim3(bs), 0, st, RA);
+        if (RA.reduced) {
+            if (RA.G <= 8 && threads * 8 * RA.ncrt < (1ll << 31))
+                hipLaunchKernelGGL(k_rns_reduce_rows<D>,
+                                   dim3((int)((threads * 8 * RA.ncrt + bs - 1) / bs)), dim3(bs), 0, st, RA);
+            else
+                hipLaunchKernelGGL(k_rns_reduce<D>, dim3((int)((threads * RA.ncrt + bs - 1) / bs)),
+                                   dim3(bs), 0, st, RA);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
