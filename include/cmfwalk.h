@@ -253,8 +253,12 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       than "rows_below" (trajectory, segment) pairs (0);
  *   "rows_below"        see lanes_per_traj (0: the single lane was faster in
  *                       every measured case);
- *   "traj_major_below"  single-lane fast walks (of at least 32 depth
- *                       segments) and RNS walks of fewer trajectories than
+ *   "traj_major_min_segments"  the depth-segment count from which a
+ *                       fast walk below "traj_major_below" trajectories is
+ *                       laid out trajectory-major (16);
+ *   "traj_major_below"  single-lane fast walks (of at least
+ *                       "traj_major_min_segments" depth segments) and RNS
+ *                       walks of fewer trajectories than
  *                       this put consecutive depth segments of one
  *                       trajectory in the lanes of a wave (one axis
  *                       sequence per wave) and combine fast segments with
