@@ -63,6 +63,10 @@ __global__ void k_rns_dentab(RnsConsts C, int np, uint32_t *dtab) {
     dtab[(size_t)k * (2 * kDenTab + 1) + i] = mpow_m2(to_mont(r, M), M);
 }
 
+// later segments of at least this many trajectory steps first re-check the
+// walk's first trajectory step (k_rns_segment)
+constexpr int kRnsProbeSteps = 16;
+
 __device__ __forceinline__ bool integer_point(double v) {
     return v == rint(v) && fabs(v) < 1073741824.0;
 }
@@ -111,6 +115,71 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
         }
         return;
     }
+    int top = 0;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
+    const uint32_t *dt = RA.dtab + (size_t)(active ? k : 0) * (2 * kDenTab + 1);
+    const uint32_t m = M.m;
+    // A walk that fails on its first trajectory step (most candidate walks at
+    // z = 1) never uses its later segments.  In RNS every failure is
+    // decided by the position alone (the lead and the step denominator of
+    // the column, the solve denominator modulo each lane), so a long later
+    // segment first re-checks the first trajectory step and stops when the
+    // walk fails there: the combine stops at segment 0, which records that
+    // failure and its step counts.
+    if (s > 0 && RA.b[s + 1] - RA.b[s] >= kRnsProbeSteps) {
+        double px[8], py[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            px[j] = xs[j];
+            py[j] = ys[j];
+        }
+        bool fail = false;
+        for (int lev = top; !fail && lev >= 1; --lev) {
+            for (int ax = wnax - 1; ax >= 0; --ax) {
+                const int dv = pick(dir, ax);
+                if (abs(dv) < lev) continue;
+                const int sign = dv > 0 ? 1 : -1;
+                double xv[8], yv[8], denom;
+                const bool inverse = axis_eval(px, py, wp, ax, sign, xv, yv, denom);
+                double col[D];
+                if (!companion_column_ref<D, PP>(xv, yv, wp, wq, A.z, col) || !isfinite(denom) ||
+                    fabs(denom) <= 64.0 * kEps) {
+                    fail = true;
+                    break;
+                }
+                if (inverse) {  // the solve denominator, as in the walk below
+                    uint32_t cr[D];
+#pragma unroll
+                    for (int i = 0; i < D; ++i) cr[i] = to_mont(to_mod_fast(col[i], M), M);
+                    const uint32_t ia = (fabs(denom) <= (double)kDenTab)
+                                            ? dt[(int)denom + kDenTab]
+                                            : mpow_m2(to_mont(to_mod_fast(denom, M), M), M);
+                    uint32_t be[D];
+                    be[D - 1] = M.r1;
+#pragma unroll
+                    for (int j = D - 2; j >= 0; --j) be[j] = msub(0u, mmul(be[j + 1], ia, M), m);
+                    uint64_t accb = 0;
+#pragma unroll
+                    for (int kk = 0; kk < D; ++kk) accb += (uint64_t)be[kk] * cr[kk];
+                    const uint32_t sd = mfma(redc(accb, M), ia, M.r1, M);
+                    if (__ballot(active && sd == 0u) & group_bits) {
+                        fail = true;
+                        break;
+                    }
+                }
+                axis_move(px, py, wp, ax, sign);
+            }
+        }
+        if (fail) {
+            if (k == 0) {
+                RA.segV[o] = 0;
+                RA.segF[o] = 0;
+                RA.segI[o] = 0;
+            }
+            return;
+        }
+    }
     const double b0 = (double)RA.b[s];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -122,11 +191,6 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) R[i][j] = (i == j) ? M.r1 : 0u;
-    int top = 0;
-#pragma unroll
-    for (int a = 0; a < 16; ++a) top = max(top, abs(dir[a]));
-    const uint32_t *dt = RA.dtab + (size_t)(active ? k : 0) * (2 * kDenTab + 1);
-    const uint32_t m = M.m;
     bool ok = true;
     int nf = 0, ni = 0;
     const int steps = RA.b[s + 1] - RA.b[s];

@@ -149,3 +149,35 @@ def test_rns_legacy_inputs_fallback(engine, orc, published):
         valid = g["valid"].astype(bool)
         assert int(valid.sum()) == run["n_valid"], (run["cmf"], run["mode"])
         assert int((valid & ~np.isnan(g["limit"])).sum()) == run["n_with_limits"], run["cmf"]
+
+
+@pytest.mark.parametrize("min_steps", [4, 48])
+def test_rns_first_step_probe(engine, orc, min_steps):
+    """The C2 shape (6F5, z=1, zeta start, directions in {0..3}^11): most
+    walks fail on their first trajectory step.  With depth segments of at
+    least 48 steps every later segment first re-checks that step and stops
+    when the walk fails there; the results stay bit-exact either way."""
+    p, q, z = 6, 5, 1.0
+    rng = np.random.default_rng(2024)
+    n = 48
+    D = rng.integers(0, 4, (n, p + q))
+    D[:8, p:] = 0  # x-only directions: these walks survive
+    D[D.sum(1) == 0, 0] = 1
+    S = np.tile(np.array([1.0] * p + [2.0] * q), (n, 1))
+    print(f"walk_rns 6F5 z=1 N=1000 rns_min_segment_steps={min_steps}", flush=True)
+    engine.set_tuning("rns_min_segment_steps", min_steps)
+    try:
+        g = _run(engine, orc, p, q, z, D, S, 1000, 50, 8)
+        a, _, _ = _rns_all(engine, p, q, z, D, S, 1000, 50, 8)
+        engine.set_tuning("rns_segments", 0)
+        b, _, _ = _rns_all(engine, p, q, z, D, S, 1000, 50, 8)
+    finally:
+        engine.set_tuning("rns_segments", 1)
+        engine.set_tuning("rns_min_segment_steps", 4)
+    assert 5 <= g["valid"].sum() < n
+    # every field of every walk, failed ones included, equals the sequential kernel's
+    assert (a["valid"] == b["valid"]).all()
+    for f in ("p", "q", "limit", "precision", "delta", "log_scale"):
+        assert _same_bits(a[f], b[f]), f
+    for f in ("n_limits", "n_fwd", "n_inv"):
+        assert (a[f] == b[f]).all(), f
