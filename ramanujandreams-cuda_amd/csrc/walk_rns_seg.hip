@@ -276,10 +276,14 @@ __global__ __launch_bounds__(256) void k_rns_segment(RnsSegArgs RA) {
                 R[i][j];
 }
 
-// W <- W * P (mod m) for P at entry stride pst, row by row.
+// W <- W * P (mod m) for P at entry stride pst, row by row (P's entries
+// loaded together first: one memory latency, not one per entry).
 template <int D>
 __device__ __forceinline__ void mat_mul_mod(uint32_t (&W)[D][D], const uint32_t *P, size_t pst,
                                             const MontMod &M) {
+    uint32_t Pm[D * D];
+#pragma unroll
+    for (int e = 0; e < D * D; ++e) Pm[e] = P[(size_t)e * pst];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         uint32_t row[D];
@@ -287,7 +291,7 @@ __device__ __forceinline__ void mat_mul_mod(uint32_t (&W)[D][D], const uint32_t 
         for (int j = 0; j < D; ++j) {
             uint64_t acc = 0;
 #pragma unroll
-            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * P[(size_t)(kk * D + j) * pst];
+            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)W[r][kk] * Pm[kk * D + j];
             row[j] = redc(acc, M);
         }
 #pragma unroll
@@ -413,12 +417,15 @@ __global__ __launch_bounds__(256) void k_rns_reduce_rows(RnsSegArgs RA) {
             break;
         }
         const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + col;
+        uint32_t Pm[D * D];
+#pragma unroll
+        for (int e = 0; e < D * D; ++e) Pm[e] = P[(size_t)e * pst];
         uint32_t row[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             uint64_t acc = 0;
 #pragma unroll
-            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)q[kk] * P[(size_t)(kk * D + j) * pst];
+            for (int kk = 0; kk < D; ++kk) acc += (uint64_t)q[kk] * Pm[kk * D + j];
             row[j] = redc(acc, M);
         }
 #pragma unroll
@@ -622,14 +629,18 @@ __global__ __launch_bounds__(256) void k_rns_combine_rows(RnsSegArgs RA) {
                 ok = false;
                 break;
             }
-            // row r of W <- W * P_s (mod m)
+            // row r of W <- W * P_s (mod m); the D^2 entries are loaded
+            // together first (one memory latency per product, not D^2)
             const uint32_t *P = RA.segP + ((size_t)s * D * D) * pst + (size_t)t * RA.np + (kact ? k : 0);
+            uint32_t Pm[D * D];
+#pragma unroll
+            for (int e = 0; e < D * D; ++e) Pm[e] = P[(size_t)e * pst];
             uint32_t row[D];
 #pragma unroll
             for (int j = 0; j < D; ++j) {
                 uint64_t acc = 0;
 #pragma unroll
-                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)w[kk] * P[(size_t)(kk * D + j) * pst];
+                for (int kk = 0; kk < D; ++kk) acc += (uint64_t)w[kk] * Pm[kk * D + j];
                 row[j] = redc(acc, M);
             }
 #pragma unroll
