@@ -8,15 +8,21 @@
 //    independent lane; a second kernel multiplies the K segment matrices in
 //    order and takes the snapshots at N-2h, N-h, N (always boundaries);
 //  * power-of-two rescaling instead of the per-step max/divide/log of
-//    normalize_projective: the max is taken every axis step (one max per
-//    entry) and the matrix is rescaled by an exact power of two only when it
-//    leaves the arithmetic's range window; the scale is an integer exponent
-//    and its log is taken once per snapshot, in FP64 (as are the snapshot's
-//    max-normalisation, p/q and delta);
+//    normalize_projective: the max is taken when a bound of the growth
+//    since the last check nears the arithmetic's range window (and at
+//    trajectory-step ends, see walk_loop), and the matrix is rescaled by an
+//    exact power of two only when it leaves that window; the scale is an
+//    integer exponent and its log is taken once per snapshot, in FP64 (as are
+//    the snapshot's max-normalisation, p/q and delta);
 //  * the theta polynomials rebuilt per axis step with fused multiply-adds,
-//    1/lead and 1/(solve denominator) taken once per step;
+//    1/lead, 1/a and 1/(solve denominator) taken once per step, the forward
+//    column pre-divided by a;
 //  * one solve denominator per inverse step (it is row-independent; the
-//    reference recomputes it per row).
+//    reference recomputes it per row);
+//  * waves whose lanes walk one direction take a loop with the direction in
+//    scalar registers (walk_loop<..., true>).
+// The limit scan at the end (finish_group, walk_common.hpp) is split over the
+// lanes of a trajectory in the row-split combine.
 // Validity parity: every failure the reference can raise depends on the
 // position only (lead, denominator, solve denominator).  For integer points
 // lead and denominator are exact in both arithmetics; a solve denominator
