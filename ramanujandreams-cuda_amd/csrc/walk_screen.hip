@@ -72,12 +72,13 @@ __global__ __launch_bounds__(1024) void k_screen_pre(WalkArgs A, int rank, int K
 
 template <int D, int PP>
 __global__ __launch_bounds__(256) void k_screen_queue(WalkArgs A, int K, int32_t *keep,
-                                                      const int32_t *queue, const int32_t *count) {
+                                                      const int32_t *queue, const int32_t *count,
+                                                      bool fwd_decided) {
     const int c = *count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
         const int t = queue[i];
         int nf, ni;
-        keep[t] = screen_traj<D, PP>(A, t, K, nf, ni) ? 1 : 0;
+        keep[t] = screen_traj<D, PP>(A, t, K, nf, ni, fwd_decided) ? 1 : 0;
     }
 }
 
@@ -114,17 +115,18 @@ static hipError_t launch_screen_d(const WalkArgs &A, int K, int32_t *keep, int32
     // phase 2 grid: enough lanes for a queue of a tenth of the candidates in
     // one pass, strided beyond that
     const int g2 = std::max(1, std::min(g, std::max(1024, g / 10)));
+    const bool fwd = screen_fwd_decided(A.p, A.q, A.z, D);
     // p known at compile time for the q = p - 1 family
     constexpr int P1 = D + 1 <= 8 ? D + 1 : 0;
     if (A.q == A.p - 1 && A.p == D)
         hipLaunchKernelGGL((k_screen_queue<D, D>), dim3(g2), dim3(bs), 0, st, A, K, keep,
-                           (const int32_t *)queue, (const int32_t *)count);
+                           (const int32_t *)queue, (const int32_t *)count, fwd);
     else if (P1 > 0 && A.q == A.p - 1 && A.p == P1)
         hipLaunchKernelGGL((k_screen_queue<D, P1>), dim3(g2), dim3(bs), 0, st, A, K, keep,
-                           (const int32_t *)queue, (const int32_t *)count);
+                           (const int32_t *)queue, (const int32_t *)count, fwd);
     else
         hipLaunchKernelGGL((k_screen_queue<D, 0>), dim3(g2), dim3(bs), 0, st, A, K, keep,
-                           (const int32_t *)queue, (const int32_t *)count);
+                           (const int32_t *)queue, (const int32_t *)count, fwd);
     return hipGetLastError();
 }
 

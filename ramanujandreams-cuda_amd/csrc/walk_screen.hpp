@@ -11,9 +11,14 @@ namespace cmfw {
 // theta_companion_column and right_mul_affine_companion,
 // cuda/cmf_pfq.cuh:146-235); nf / ni count the evaluated forward / inverse
 // axis steps, the failing one included.  Start points off the integer grid
-// (or beyond 2^23) pass with no step evaluated.
+// (or beyond 2^23) pass with no step evaluated.  fwd_decided: the walk
+// passed screen_int's checks of the same K steps and the lead cannot vanish
+// otherwise (screen_fwd_decided), so a forward step cannot fail here (its
+// lead and denominator are nonzero integers or a constant lead, and its column
+// is finite at these magnitudes); only the inverse steps are evaluated.
 template <int D, int PP = 0>
-__device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
+__device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni,
+                            bool fwd_decided = false) {
 #pragma clang fp contract(off)
     // PP > 0: PP F (PP-1) known at compile time
     const int wp = PP > 0 ? PP : A.p, wq = PP > 0 ? PP - 1 : A.q;
@@ -43,6 +48,10 @@ __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
                 const bool inverse = axis_eval(xs, ys, wp, ax, sign, xv, yv, denom);
                 nf += inverse ? 0 : 1;
                 ni += inverse ? 1 : 0;
+                if (fwd_decided && !inverse) {
+                    axis_move(xs, ys, wp, ax, sign);
+                    continue;
+                }
                 double col[D];
                 ok = companion_column_ref<D, PP>(xv, yv, wp, wq, A.z, col) && isfinite(denom) &&
                      !(fabs(denom) <= 64.0 * kEps);
@@ -65,6 +74,19 @@ __device__ bool screen_traj(const WalkArgs &A, int t, int K, int &nf, int &ni) {
         }
     }
     return ok;
+}
+
+// Whether a walk that passes screen_int has no forward step that can fail
+// (theta_companion_column's lead test, cuda/cmf_pfq.cuh:146-181): at z = 1
+// with p = q + 1 the lead is the integer screen_int checks; for p < q + 1 it
+// is 1; for p = q + 1 it is the constant 1 - z; for p > q + 1 it is -z,
+// tested against 64 eps (1 + sum of |coordinates|) < 1e-3 on the integer
+// grid below 2^23 (at most 16 coordinates).
+inline bool screen_fwd_decided(int p, int q, double z, int rank) {
+    if (z == 1.0 && p == q + 1 && rank == q) return true;
+    if (p < q + 1) return true;
+    if (p == q + 1) return fabs(1.0 - z) > 64.0 * kEps * fmax(1.0, fabs(1.0 - z));
+    return fabs(z) >= 1e-3;
 }
 
 // Exact integer part of screen_traj: the step denominators and, where the

@@ -67,7 +67,7 @@ def test_screen_keeps_off_grid_starts(engine):
 
 
 @pytest.mark.parametrize("p,q,z", [(2, 1, 0.5), (3, 2, 0.5), (4, 3, 1.0), (6, 5, 1.0),
-                                   (8, 7, 1.0), (5, 3, 1.0), (3, 3, 0.5)])
+                                   (8, 7, 1.0), (5, 3, 1.0), (3, 3, 0.5), (5, 3, 0.5)])
 def test_screen_two_phase_equals_one_pass(engine, p, q, z):
     """The two-phase screen (exact integer lead / denominator checks, then the
     FP64 checks for the undecided) keeps exactly what the one-pass screen
@@ -91,3 +91,28 @@ def test_screen_two_phase_equals_one_pass(engine, p, q, z):
             assert 0 < int(two.sum()) < n
     finally:
         engine.set_tuning("screen_two_phase", 1)
+
+
+@pytest.mark.parametrize("p,q,z", [(5, 3, 0.0), (5, 3, 1e-5), (4, 3, 1.0 - 1e-15), (4, 3, 1.0 + 2e-15)])
+def test_screen_two_phase_vanishing_constant_lead(engine, p, q, z):
+    """Where the lead is a constant that the reference's test finds (nearly)
+    zero (-z for p > q + 1, 1 - z for p = q + 1 and z != 1), forward steps
+    fail too: the second phase must evaluate them, and the two-phase screen
+    keeps what the one-pass screen keeps."""
+    rng = np.random.default_rng(11 * p + q)
+    n = 4000
+    D = rng.integers(-3, 4, (n, p + q))
+    D[np.abs(D).sum(1) == 0, 0] = 1
+    D[: n // 4, p:] = 0                                # forward x steps only
+    D[: n // 4, :p] = np.abs(D[: n // 4, :p])
+    D[: n // 4, 0] = np.maximum(D[: n // 4, 0], 1)
+    S = rng.integers(1, 6, (n, p + q)).astype(float)
+    print(f"two-phase screen, constant lead {p}F{q} z={z!r}", flush=True)
+    try:
+        engine.set_tuning("screen_two_phase", 0)
+        one = engine.screen(p, q, z, D, S, 1).cpu().numpy()
+        engine.set_tuning("screen_two_phase", 1)
+        two = engine.screen(p, q, z, D, S, 1).cpu().numpy()
+    finally:
+        engine.set_tuning("screen_two_phase", 1)
+    assert np.array_equal(one, two), int((one != two).sum())
