@@ -118,26 +118,35 @@ __device__ int screen_int(const WalkArgs &A, int rank, int t, int K) {
     }
     if (!clean) return 2;
     const bool lead_moves = A.z == 1.0 && A.p == A.q + 1 && rank == A.q;
+    // Levels from the wave's largest |direction| down (a lane's levels above
+    // its own largest move nothing), every axis evaluated with its step
+    // predicated on |direction| >= level: straight-line code instead of a
+    // branch per axis.  Whether a step fails is all that is kept, so the
+    // steps after a failure may run on.
+    int wtop = 0;
+    while (__any(top > wtop)) ++wtop;
+    bool fail = false;
     for (int step = 1; step <= K; ++step) {
-        for (int lev = top; lev >= 1; --lev) {
+        for (int lev = wtop; lev >= 1; --lev) {
 #pragma unroll
             for (int ax = NAX - 1; ax >= 0; --ax) {
                 const int dv = dir[ax];
-                if (abs(dv) < lev) continue;
+                const bool act = abs(dv) >= lev;
                 const int sign = dv > 0 ? 1 : -1;
                 const bool isx = ax < A.p;
                 const bool inverse = isx ? (sign < 0) : (sign > 0);
                 const int v = pos[ax];
                 const int den = isx ? (inverse ? v - 1 : v) : (inverse ? v : v - 1);
                 const int lead = (sy + ((!isx && inverse) ? 1 : 0)) - (sx - ((isx && inverse) ? 1 : 0));
-                if (den == 0 || (lead_moves && lead == 0)) return 0;
-                pos[ax] += sign;
-                sx += isx ? sign : 0;
-                sy += isx ? 0 : sign;
+                fail = fail || (act && (den == 0 || (lead_moves && lead == 0)));
+                const int mv = act ? sign : 0;
+                pos[ax] += mv;
+                sx += isx ? mv : 0;
+                sy += isx ? 0 : mv;
             }
         }
     }
-    return 1;
+    return fail ? 0 : 1;
 }
 
 }  // namespace cmfw
