@@ -216,7 +216,8 @@ def plain_6f5(torch, cw, np, reps=3):
     6F5 walk of this shape stays valid to N=2000).  Canonical flops of the
     executed axis steps over the walk's time (HIP events on the stream the
     kernels run on) and over the segment kernel's own time, for the fast and
-    the reference-order engines.
+    the reference-order engines, and the fast engine with 8 depth segments
+    (fast_segments) instead of the default plan's ~128.
     c2: 6F5, z=1, zeta start, directions in {0..3}^11, 10,000 trajectories,
     N=1000, h=50, walked f64 (reference order and fast) and RNS (8 moduli)."""
     p, q = 6, 5
@@ -262,6 +263,20 @@ def plain_6f5(torch, cw, np, reps=3):
                 "kernel": "k_walk_seg<6,f64>", "avg_launch_ms": seg,
                 "tflops": fl / (seg * 1e-3) / 1e12,
                 "frac_fp64_peak": fl / (seg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    # the same walk with 8 depth segments instead of about 128 (the default
+    # plan fills the GPU for small batches; a batch this large fills it with
+    # fewer, longer segments and a shorter combine)
+    out = cw.alloc_results(n)
+    cw.set_tuning("fast_segments", 8)
+    try:
+        ms = timed(lambda: cw.walk_f64_fast(p, q, z, dd, ss, N, h, out=out))
+    finally:
+        cw.set_tuning("fast_segments", 128)
+    r = cw.results_to_numpy(out)
+    fl = float(r["n_fwd"].astype(np.float64).sum() * ff + r["n_inv"].astype(np.float64).sum() * fi)
+    walk["walk_f64_fast_8_segments"] = {"ms": ms, "traj_per_s": n / (ms * 1e-3), "valid": int(r["valid"].sum()),
+                                        "tflops": fl / (ms * 1e-3) / 1e12,
+                                        "frac_fp64_peak": fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     n, N, h, z = 10000, 1000, 50, 1.0
     D, S = cw.synth_inputs(42, n, p, q, 2, 1, 0, 3)
     dd, ss = cw.to_device_inputs(D[:, :p + q], S[:, :p + q])
