@@ -267,6 +267,11 @@ int cmfw_synth_mixed(uint64_t seed, int64_t n, int32_t cmf_lo, int32_t cmf_hi, i
  *                       products of each CRT interval in parallel before
  *                       the sequential combine; 0: the combine takes every
  *                       segment (same residues);
+ *   "pool_keep_bytes"   device memory the library's stream-ordered pool
+ *                       (one per device; temporaries of calls given no
+ *                       workspace) keeps mapped after host synchronisations
+ *                       (-1, the default: all it has held; 0 returns
+ *                       it at each);
  *   "kernel_events"     1: record kernel timing events (cmfw_kernel_time)
  *                       (0);
  *   "screen_two_phase"  cmfw_screen without step counts: 1 (default) the

@@ -1,9 +1,9 @@
 // capi.hip -- the extern "C" boundary of libcmfwalk.so (include/cmfwalk.h).
 //
 // Validates arguments on the host, lays out temporary snapshot storage in
-// the caller's workspace (or in stream-ordered memory from hipMallocAsync),
-// and launches the kernels on the caller's stream.  Nothing here blocks the
-// host on the device.
+// the caller's workspace (or in stream-ordered memory from a pool of the
+// library's own), and launches the kernels on the caller's stream.  Nothing
+// here blocks the host on the device.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -116,6 +116,46 @@ int g_rns_min_segment_steps = 4;
 int g_collapse_log2 = 40;
 bool g_rns_segments = true;
 
+// Stream-ordered temporaries come from a memory pool of the library's own,
+// one per device.  Its release threshold (g_pool_keep bytes, default all)
+// keeps freed blocks mapped across host synchronisations; a pool that
+// returns them at every synchronisation (the device's default pool does)
+// maps its memory again on the next call, a host stall of up to a few
+// hundred microseconds before a large walk.
+std::mutex g_pool_mu;
+std::map<int, hipMemPool_t> g_pools;
+uint64_t g_pool_keep = ~0ull;
+
+hipError_t pool_malloc(void **p, size_t bytes, hipStream_t st) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipMemPool_t pool = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_pools.find(dev);
+        if (it == g_pools.end()) {
+            hipMemPoolProps props;
+            memset(&props, 0, sizeof props);
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            e = hipMemPoolCreate(&pool, &props);
+            if (e != hipSuccess) return e;
+            uint64_t keep = g_pool_keep;
+            e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            if (e != hipSuccess) {
+                (void)hipMemPoolDestroy(pool);
+                return e;
+            }
+            g_pools[dev] = pool;
+        } else {
+            pool = it->second;
+        }
+    }
+    return hipMallocFromPoolAsync(p, bytes, pool, st);
+}
+
 // Depth segments of a fast walk: multiples of a segment length L plus the
 // snapshot depths N-2h, N-h, N; b[0] = 0, b[nseg] = N.  L depends on N
 // alone (about g_fast_segments segments, at least g_min_segment_steps
@@ -161,8 +201,8 @@ int get_snaps(int D, int n, cmfw_snapshots *snaps, void *ws, size_t ws_bytes, hi
             return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", ws_bytes, need);
         base = ws;
     } else {
-        hipError_t e = hipMallocAsync(&base, need, st);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(snapshots)");
+        hipError_t e = pool_malloc((void **)&base, need, st);
+        if (e != hipSuccess) return hip_fail(e, "pool alloc(snapshots)");
         S.owned = base;
     }
     size_t m = (size_t)3 * D * D * (size_t)n * sizeof(double);
@@ -344,8 +384,8 @@ static int walk_fast_common(bool ff, const cmfw_desc *desc, const cmfw_inputs *i
         seg = (char *)workspace + sb;
     } else {
         if ((rc = get_snaps(desc->rank, n, snaps, nullptr, 0, st, S))) return rc;
-        hipError_t e = hipMallocAsync(&owned, seg_need, st);
-        if (e != hipSuccess) return release(S, st, hip_fail(e, "hipMallocAsync(segments)"));
+        hipError_t e = pool_malloc((void **)&owned, seg_need, st);
+        if (e != hipSuccess) return release(S, st, hip_fail(e, "pool alloc(segments)"));
         seg = owned;
     }
     cmfw::WalkArgs A;
@@ -530,8 +570,8 @@ int cmfw_walk_rns(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int32
         seg = (char *)workspace + sb;
     } else {
         if ((rc = get_snaps(desc->rank, n, snaps, nullptr, 0, st, S))) return rc;
-        hipError_t e = hipMallocAsync(&owned, seg_need, st);
-        if (e != hipSuccess) return release(S, st, hip_fail(e, "hipMallocAsync(rns segments)"));
+        hipError_t e = pool_malloc((void **)&owned, seg_need, st);
+        if (e != hipSuccess) return release(S, st, hip_fail(e, "pool alloc(rns segments)"));
         seg = owned;
     }
     fill_args(A, desc, in, N, h, out, S);
@@ -563,8 +603,8 @@ int cmfw_screen(const cmfw_desc *desc, const cmfw_inputs *in, int32_t K, int32_t
     hipStream_t st = (hipStream_t)stream;
     void *ws = nullptr;
     if (!steps && g_screen_two_phase) {
-        hipError_t e = hipMallocAsync(&ws, cmfw::screen_queue_bytes((int)in->n_traj), st);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(screen queue)");
+        hipError_t e = pool_malloc((void **)&ws, cmfw::screen_queue_bytes((int)in->n_traj), st);
+        if (e != hipSuccess) return hip_fail(e, "pool alloc(screen queue)");
     }
     hipError_t e = cmfw::launch_screen(desc->rank, A, K, keep, steps, ws, st);
     rc = (e == hipSuccess) ? CMFW_OK : hip_fail(e, "screen launch");
@@ -593,8 +633,8 @@ int cmfw_mlp_forward(const float *params, const uint16_t *params_bf16, const uin
     if (ws && workspace_bytes < need)
         return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes, need);
     if (!ws) {
-        hipError_t e = hipMallocAsync(&owned, need, st);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(mlp)");
+        hipError_t e = pool_malloc((void **)&owned, need, st);
+        if (e != hipSuccess) return hip_fail(e, "pool alloc(mlp)");
         ws = owned;
     }
     hipError_t e = cmfw::mlp_forward(params, params_bf16, X, batch, y, ws, st);
@@ -614,8 +654,8 @@ int cmfw_mlp_grad(const float *params, const uint16_t *params_bf16, const uint16
     if (ws && workspace_bytes < need)
         return fail(CMFW_ENOMEM, "workspace %zu bytes < %zu needed", workspace_bytes, need);
     if (!ws) {
-        hipError_t e = hipMallocAsync(&owned, need, st);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(mlp)");
+        hipError_t e = pool_malloc((void **)&owned, need, st);
+        if (e != hipSuccess) return hip_fail(e, "pool alloc(mlp)");
         ws = owned;
     }
     hipError_t e = cmfw::mlp_grad(params, params_bf16, X, target, batch, grad, sq_rows, ws, st);
@@ -708,6 +748,17 @@ int cmfw_set_tuning(const char *key, int64_t value) {
         g_traj_major_below = value;
         return CMFW_OK;
     }
+    if (!strcmp(key, "pool_keep_bytes")) {
+        if (value < -1) return fail(CMFW_EINVAL, "pool_keep_bytes must be >= -1");
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        g_pool_keep = value < 0 ? ~0ull : (uint64_t)value;
+        for (auto &kv : g_pools) {
+            uint64_t keep = g_pool_keep;
+            hipError_t e = hipMemPoolSetAttribute(kv.second, hipMemPoolAttrReleaseThreshold, &keep);
+            if (e != hipSuccess) return hip_fail(e, "pool_keep_bytes");
+        }
+        return CMFW_OK;
+    }
     if (!strcmp(key, "kernel_events")) {
         if (value < 0 || value > 1) return fail(CMFW_EINVAL, "kernel_events must be 0 or 1");
         g_kernel_events = value != 0;
@@ -779,8 +830,8 @@ int cmfw_walk_exact(const cmfw_desc *desc, const cmfw_inputs *in, int32_t N, int
     std::vector<char> host(mb);
     cmfw::exact_fill_mods(primes, K, host.data());
     void *mods = nullptr;
-    hipError_t e = hipMallocAsync(&mods, mb, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync(moduli)");
+    hipError_t e = pool_malloc((void **)&mods, mb, st);
+    if (e != hipSuccess) return hip_fail(e, "pool alloc(moduli)");
     e = hipMemcpyAsync(mods, host.data(), mb, hipMemcpyHostToDevice, st);
     // the host table must outlive the copy
     if (e == hipSuccess) e = hipStreamSynchronize(st);

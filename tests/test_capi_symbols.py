@@ -46,6 +46,11 @@ def test_host_helpers():
     assert cmfwalk.modulus(64) == 0
     L = cmfwalk.load_library()
     assert L.cmfw_workspace_bytes(cmfwalk.make_desc(8, 7, 0.5), 1000, 1000, 50, 8) >= 3 * 64 * 1000 * 8
+    # the pool knob: -1 (keep all, the default) and byte counts >= 0
+    cmfwalk.set_tuning("pool_keep_bytes", 0)
+    cmfwalk.set_tuning("pool_keep_bytes", -1)
+    with pytest.raises(cmfwalk.CmfwError):
+        cmfwalk.set_tuning("pool_keep_bytes", -2)
 
 
 def test_moduli_agree_with_oracle(orc):

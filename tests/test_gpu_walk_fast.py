@@ -263,3 +263,33 @@ def test_traj_major_equals_seg_major(engine, walk, p, q, z):
     assert (a["valid"] == 1).sum() > n // 10
     for fld in a.dtype.names:
         assert np.array_equal(a[fld], b[fld], equal_nan=True), fld
+
+
+def test_pool_keep_bytes_same_results(engine):
+    """Temporaries from the library's pool: results are bit-identical whether
+    the pool keeps its memory across synchronisations (default) or returns
+    it at each (pool_keep_bytes 0), over calls separated by host syncs."""
+    import torch
+    p, q, z = 6, 5, 1.0
+    n = 5000
+    rng = np.random.default_rng(7)
+    D = rng.integers(0, 3, (n, p + q))
+    D[:, 0] = np.maximum(D[:, 0], 1)
+    S = np.tile(np.array([1.0] * p + [2.0] * q), (n, 1))
+    N, h = 300, 30
+    outs = []
+    for keep in (0, 1 << 40, 0):
+        print(f"pool_keep_bytes={keep}", flush=True)
+        engine.set_tuning("pool_keep_bytes", keep)
+        try:
+            for walk in ("walk_f64_fast", "walk_rns"):
+                outs.append((walk, engine.results_to_numpy(getattr(engine, walk)(p, q, z, D, S, N, h))))
+                torch.cuda.synchronize()
+        finally:
+            engine.set_tuning("pool_keep_bytes", -1)
+    for walk in ("walk_f64_fast", "walk_rns"):
+        rs = [o for w, o in outs if w == walk]
+        for r in rs[1:]:
+            for k in r.dtype.names:
+                a, b = np.asarray(rs[0][k]), np.asarray(r[k])
+                assert np.array_equal(a, b, equal_nan=a.dtype.kind == "f"), (walk, k)
