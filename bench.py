@@ -115,9 +115,14 @@ def top_with_ties(torch, score, k, cap):
     n = score.numel()
     if n <= k:
         return torch.topk(score, n).indices
-    tau = torch.topk(score, k).values[-1]
-    m = min(cap, int((score >= tau).sum().item()))
-    return torch.topk(score, m).indices
+    if cap <= k:
+        return torch.topk(score, cap).indices
+    # one top-cap selection: the scores >= the k-th best are a prefix of it
+    # (all of them when they number at most cap, else the cap best), so only
+    # the prefix length waits for the device
+    top = torch.topk(score, min(cap, n))
+    m = min(cap, int((top.values >= top.values[k - 1]).sum().item()))
+    return top.indices[:m]
 
 
 class Search:
