@@ -148,6 +148,12 @@ hipError_t pool_malloc(void **p, size_t bytes, hipStream_t st) {
                 (void)hipMemPoolDestroy(pool);
                 return e;
             }
+            // a block freed on another stream is reused once that free has
+            // completed, never by making this stream wait for the other one
+            // (calls on two streams of one device stay independent; the pool
+            // grows instead).  Advisory: the pool is correct either way.
+            int no_wait = 0;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no_wait);
             g_pools[dev] = pool;
         } else {
             pool = it->second;
