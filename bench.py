@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="time searches one at a time only (value = the unpipelined rate)")
     ap.add_argument("--workload", choices=["search", "batch"], default="search",
                     help="search: the headline 8F7 search; batch: the mixed-batch walk "
                          "(3F2..8F7, z in {0.5,1}, N in [200,5000], h=50)")
@@ -501,20 +503,22 @@ def main():
     cw.set_tuning("kernel_events", 0)
     ktimes = {k: cw.kernel_time(k) for k in ("screen_pre", "walk_seg_f64", "rns_segment")}
     # ---- the same searches back to back, software-pipelined over two streams
-    s2 = torch.cuda.Stream()
-    srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.warmup, s2)
-    torch.cuda.synchronize()
-    t_pp = time.perf_counter()
-    out_pp = srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.steps, s2)
-    torch.cuda.synchronize()
-    sec_pp = (time.perf_counter() - t_pp) / args.steps
-    # every pipelined search reports the unpipelined search's results
-    for k in ("live", "cand", "top"):
-        assert torch.equal(out_pp[k], out_p[k]), k
-    bits = lambda x: x.view(torch.int64) if x.dtype == torch.float64 else x  # noqa: E731
-    for k in ("valid", "n_limits", "delta", "limit"):
-        assert torch.equal(bits(out_pp["rns"][k]), bits(out_p["rns"][k])), k
-    log(f"proposal f64, pipelined over two streams: {sec_pp * 1e3:.3f} ms per search")
+    sec_pp = sec_p
+    if not args.no_pipeline:
+        s2 = torch.cuda.Stream()
+        srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.warmup, s2)
+        torch.cuda.synchronize()
+        t_pp = time.perf_counter()
+        out_pp = srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.steps, s2)
+        torch.cuda.synchronize()
+        sec_pp = (time.perf_counter() - t_pp) / args.steps
+        # every pipelined search reports the unpipelined search's results
+        for k in ("live", "cand", "top"):
+            assert torch.equal(out_pp[k], out_p[k]), k
+        bits = lambda x: x.view(torch.int64) if x.dtype == torch.float64 else x  # noqa: E731
+        for k in ("valid", "n_limits", "delta", "limit"):
+            assert torch.equal(bits(out_pp["rns"][k]), bits(out_p["rns"][k])), k
+        log(f"proposal f64, pipelined over two streams: {sec_pp * 1e3:.3f} ms per search")
     st_p = T_p.stages(names_p)
     log("proposal f64 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_p.items()}))
     # ---- the same pipeline with two-float walks
@@ -593,9 +597,10 @@ def main():
                                f"{args.n_cand} candidates (mt19937_64 seed 42)",
                    "pipeline": f"proposal: screen, 100-step f64 feature walk, MLP rank, top "
                                f"{args.deep_m} walked f64, top 100 + ties (<= {args.verify_v}) RNS-verified "
-                               f"({args.n_primes} moduli); consecutive searches pipelined over two "
-                               f"streams (search i's RNS verification beside search i+1's screen "
-                               f"through selection)",
+                               f"({args.n_primes} moduli)" +
+                               ("" if args.no_pipeline else
+                                "; consecutive searches pipelined over two streams (search i's RNS "
+                                "verification beside search i+1's screen through selection)"),
                    "parallelism": "single-gpu"},
         "speedup_vs_plain_f64": sec_b / sec_p,
         "recall_at_100": rec_p,
