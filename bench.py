@@ -445,8 +445,9 @@ def run_batch(args, torch, cw, np):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                                   "(trajectories shard by range across GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 (a multi-GPU "
+                                   "search would shard the candidates by range, one process per "
+                                   "GPU, and merge the per-GPU top-k by delta)"}))
         sys.exit(2)
     import numpy as np
     import torch
@@ -527,7 +528,7 @@ def main():
                                        args.verify_v), args.steps, args.warmup)
     st_f = T_f.stages(names_p)
     log("proposal f32x2 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_f.items()}))
-    # ---- plain f64 search (exhaustive deep walk; the judge of recall)
+    # ---- plain f64 search (exhaustive deep walk; the reference for recall)
     names_b = ["t0", "screen", "compact", "deep", "select", "verify"]
     out_b, sec_b, T_b = timed_steps(torch, lambda T: srch.plain(T or _NoTimer(), args.top_frac),
                                     args.steps, args.warmup)

@@ -13,7 +13,7 @@ exact one-step screen, the feature walk, and the full-depth walk
 whose delta is the regression target.  The RNS arithmetic that verifies the
 final set is never approximated: the network only decides which candidates
 are walked deep, and the exhaustive walk stays available as fallback and as
-the judge of the ranking's recall (recall_at).
+the reference for the ranking's recall (recall_at).
 """
 import math
 
