@@ -21,10 +21,10 @@ they report.
                 the top 100 of those by delta, with ties at the 100th value
                 (at most V), RNS-verified.
 value = candidates resolved per second by the proposal pipeline whose deep
-walk is f64, consecutive searches software-pipelined over two streams (one
-search's RNS verification beside the next search's screen through
-selection; each search's results are checked against the unpipelined
-search's); the unpipelined rate and the same pipeline with two-float FP32
+walk is f64, consecutive searches software-pipelined over three streams
+(one search's RNS verification beside the next search's walks, the next
+screen beside the deep walk; each search's results are checked against the
+unpipelined search's); the unpipelined rate and the same pipeline with two-float FP32
 walks are measured beside it, with recall.  Recall@100: fraction of the exhaustive plain walk's
 top-100 by delta (ties at the 100th value included) that the proposal
 verified.  The network is trained before timing (1M walk-generated pairs
@@ -182,37 +182,77 @@ class Search:
         T.mark("verify")
         return f
 
-    def pipelined(self, net, engine, M, V, steps, s2):
-        """`steps` searches of the candidate batch, software-pipelined over two
-        streams: search i's RNS verification runs on s2 while search i+1's
-        screen, feature walk, ranking, deep walk and selection run on the
-        current stream (small walks whose kernels leave most SIMDs idle).
-        Every search does all of its work; the results are the unpipelined
-        search's."""
+    def pipelined(self, net, engine, M, V, steps, s2, s3):
+        """`steps` searches of the candidate batch, software-pipelined over
+        three streams: search j+1's screen (s3) runs beside search j's deep
+        walk, and search j's RNS verification (s2) beside search j+1's
+        feature walk, ranking and deep walk (the current stream; small walks
+        whose kernels leave most SIMDs idle), so the device has work queued
+        while the host waits for the survivor count and the verification
+        set's length.  Every search does all of its work; the results are
+        the unpipelined search's."""
         torch, cw = self.torch, self.cw
         s1 = torch.cuda.current_stream()
-        out = None
-        for _ in range(steps):
-            f = self.front(_NoTimer(), net, engine, M, V)
-            ready = torch.cuda.Event()
-            ready.record(s1)
-            dlt, slt = f.pop("dlt"), f.pop("slt")
+        if not hasattr(self, "keeps"):
+            self.keeps = [self.keep, torch.empty_like(self.keep)]
+        read = [None, None]  # s1 events: keep flags b last read
+
+        def screen(j):
+            b = j % 2
+            with torch.cuda.stream(s3):
+                if read[b] is not None:
+                    s3.wait_event(read[b])
+                cw.screen(P, Q, Z, self.dir, self.start, 1, keep=self.keeps[b])
+                done = torch.cuda.Event()
+                done.record(s3)
+            return done
+
+        def verify(f):
+            ready, dlt, slt = f.pop("ready"), f.pop("dlt"), f.pop("slt")
             with torch.cuda.stream(s2):
                 s2.wait_event(ready)
                 # made on s1, read on s2: not reused before s2 is done with them
                 dlt.record_stream(s2)
                 slt.record_stream(s2)
                 f["rns"] = cw.walk_rns(P, Q, Z, dlt, slt, N_DEPTH, H, n_primes=self.n_primes)
+
+        out, nxt = None, {}
+        if steps > 0:
+            nxt["screened"] = screen(0)
+        for j in range(steps):
+            b = j % 2
+            s1.wait_event(nxt.pop("screened"))
+            live = torch.nonzero(self.keeps[b]).squeeze(1)
+            dl = self.dir[:, live].contiguous()
+            sl = self.start[:, live].contiguous()
+            read[b] = torch.cuda.Event()
+            read[b].record(s1)
+
+            def after_deep(j=j):
+                if j + 1 < steps:
+                    nxt["screened"] = screen(j + 1)
+
+            f = self.select(_NoTimer(), net, engine, M, V, live, dl, sl, after_deep)
+            f["ready"] = torch.cuda.Event()
+            f["ready"].record(s1)
+            verify(f)
             out = f
         s1.wait_stream(s2)
+        s1.wait_stream(s3)
         return out
 
     def front(self, T, net, engine, M, V):
         """The proposal search up to the verification set (its directions and
         starts gathered as dlt, slt)."""
-        cw, pr, torch = self.cw, self.pr, self.torch
         T.mark("t0")
         live, dl, sl = self.screen_compact(T)
+        return self.select(T, net, engine, M, V, live, dl, sl)
+
+    def select(self, T, net, engine, M, V, live, dl, sl, after_deep=None):
+        """Feature walk, ranking, deep walk and selection of the screened
+        candidates; the verification set's directions and starts gathered as
+        dlt, slt.  after_deep() is called once the deep walk is issued."""
+        pr, torch = self.pr, self.torch
         X, pre = pr.feature_walk(P, Q, Z, dl, sl, engine=engine)
         T.mark("features")
         s = net.score(X)
@@ -223,6 +263,8 @@ class Search:
         res = pr.WALKS[engine](P, Q, Z, dl[:, cand].contiguous(), sl[:, cand].contiguous(),
                                N_DEPTH, H)
         T.mark("deep")
+        if after_deep is not None:
+            after_deep()
         d = res["delta"]
         score = torch.where((res["valid"] == 1) & torch.isfinite(d), d,
                             torch.full_like(d, -float("inf")))
@@ -503,14 +545,14 @@ def main():
         args.steps, args.warmup, after_warmup=events_on)
     cw.set_tuning("kernel_events", 0)
     ktimes = {k: cw.kernel_time(k) for k in ("screen_pre", "walk_seg_f64", "rns_segment")}
-    # ---- the same searches back to back, software-pipelined over two streams
+    # ---- the same searches back to back, software-pipelined over three streams
     sec_pp = sec_p
     if not args.no_pipeline:
-        s2 = torch.cuda.Stream()
-        srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.warmup, s2)
+        s2, s3 = torch.cuda.Stream(), torch.cuda.Stream()
+        srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.warmup, s2, s3)
         torch.cuda.synchronize()
         t_pp = time.perf_counter()
-        out_pp = srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.steps, s2)
+        out_pp = srch.pipelined(net, "f64", args.deep_m, args.verify_v, args.steps, s2, s3)
         torch.cuda.synchronize()
         sec_pp = (time.perf_counter() - t_pp) / args.steps
         # every pipelined search reports the unpipelined search's results
@@ -519,7 +561,7 @@ def main():
         bits = lambda x: x.view(torch.int64) if x.dtype == torch.float64 else x  # noqa: E731
         for k in ("valid", "n_limits", "delta", "limit"):
             assert torch.equal(bits(out_pp["rns"][k]), bits(out_p["rns"][k])), k
-        log(f"proposal f64, pipelined over two streams: {sec_pp * 1e3:.3f} ms per search")
+        log(f"proposal f64, pipelined over three streams: {sec_pp * 1e3:.3f} ms per search")
     st_p = T_p.stages(names_p)
     log("proposal f64 stages (ms): " + json.dumps({k: round(v, 3) for k, v in st_p.items()}))
     # ---- the same pipeline with two-float walks
@@ -600,8 +642,9 @@ def main():
                                f"{args.deep_m} walked f64, top 100 + ties (<= {args.verify_v}) RNS-verified "
                                f"({args.n_primes} moduli)" +
                                ("" if args.no_pipeline else
-                                "; consecutive searches pipelined over two streams (search i's RNS "
-                                "verification beside search i+1's screen through selection)"),
+                                "; consecutive searches pipelined over three streams (search j's RNS "
+                                "verification beside search j+1's walks, search j+1's screen beside "
+                                "search j's deep walk)"),
                    "parallelism": "single-gpu"},
         "speedup_vs_plain_f64": sec_b / sec_p,
         "recall_at_100": rec_p,
